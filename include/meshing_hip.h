@@ -1,0 +1,111 @@
+/* meshing_hip.h -- C ABI of libmeshing_hip.so, MI355X (gfx950) isosurface
+ * extraction: Marching Cubes, Marching Tetrahedra, Naive Surface Nets.
+ *
+ * Drop-in boundary: the reference's generic function
+ *     isosurface(sdf::AbstractArray{T,3}, method, X=-1:1, Y=-1:1, Z=-1:1)
+ *         -> (Vector{NTuple{3,float(FT)}}, Vector{NTuple{3,Int}})
+ * (Meshing.jl v0.7.0 src/isosurface.jl:30-32, src/marching_cubes.jl:27-64,
+ * src/marching_tetrahedra.jl:129-163, src/algorithmtypes.jl:10,23-25,37-40).
+ * A Julia `ccall` (or Python ctypes) wrapper computes first(X)/last(X) and
+ * the Julia types of iso/eps/coordinates, calls msh_count to learn the output
+ * sizes, allocates, and calls msh_extract.  Results are identical to the
+ * reference's: same vertex and face order, 1-based Int64 faces, vertex
+ * coordinates rounded to float(FT) after the same mixed-precision arithmetic.
+ *
+ * All functions return an msh_status; on error msh_last_error(ctx) holds a
+ * message.  The library never aborts.  A context owns one HIP stream (or
+ * uses the caller's, see msh_set_stream) and grow-only device workspaces;
+ * a context must not be used by two threads at once.  Separate contexts are
+ * independent.
+ */
+#ifndef MESHING_HIP_H
+#define MESHING_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  MSH_OK = 0,
+  MSH_ERR_ARGUMENT = 1,    /* Julia ArgumentError / MethodError analogue */
+  MSH_ERR_OUT_OF_MEMORY = 2,
+  MSH_ERR_DEVICE = 3,      /* HIP runtime failure */
+  MSH_ERR_STATE = 4,       /* msh_extract without a matching msh_count */
+  MSH_ERR_UNSUPPORTED = 5  /* element type the GPU path does not handle */
+} msh_status;
+
+/* AbstractMeshingAlgorithm subtypes (src/algorithmtypes.jl:10). */
+typedef enum {
+  MSH_MARCHING_CUBES = 0,       /* MarchingCubes{T}(iso)           :23-25 */
+  MSH_MARCHING_TETRAHEDRA = 1,  /* MarchingTetrahedra{T,E}(iso,eps) :37-40 */
+  MSH_NAIVE_SURFACE_NETS = 2    /* named by the reference, no code in v0.7.0 */
+} msh_method;
+
+/* Julia scalar types that take part in promotion. */
+typedef enum { MSH_INT64 = 0, MSH_FLOAT32 = 1, MSH_FLOAT64 = 2 } msh_kind;
+
+/* Where a buffer lives. */
+typedef enum { MSH_HOST = 0, MSH_DEVICE = 1 } msh_memory;
+
+/* The sampled field `sdf`: column-major (x fastest), size(sdf) = (nx,ny,nz). */
+typedef struct {
+  int32_t dtype;   /* eltype(sdf): MSH_FLOAT32 or MSH_FLOAT64 */
+  int32_t memory;  /* MSH_HOST or MSH_DEVICE */
+  int64_t nx, ny, nz;
+  const void* data;
+} msh_field;
+
+/* The method object and the X, Y, Z axis arguments. */
+typedef struct {
+  int32_t method;              /* msh_method */
+  int32_t iso_kind;            /* typeof(method.iso) */
+  int32_t eps_kind;            /* typeof(method.eps), MT only */
+  int32_t axis_kind[3];        /* eltype(first(X)), eltype(first(Y)), eltype(first(Z)) */
+  double iso, eps;             /* value when the kind is a float */
+  int64_t iso_int, eps_int;    /* value when the kind is MSH_INT64 */
+  double first[3], last[3];    /* first(X), last(X), ... (Int endpoints as Float64) */
+} msh_params;
+
+/* Context life cycle.  `device` is a HIP device ordinal. */
+int msh_create(int device, void** ctx);
+int msh_destroy(void* ctx);
+/* Run subsequent work on the caller's hipStream_t; NULL is HIP's default
+ * (null) stream.  A new context uses a non-blocking stream of its own. */
+int msh_set_stream(void* ctx, void* hip_stream);
+int msh_synchronize(void* ctx);
+const char* msh_last_error(const void* ctx);
+
+/* float(FT) of the result (src/marching_cubes.jl:31,33; src/marching_tetrahedra.jl:131,135):
+ * MSH_FLOAT32 or MSH_FLOAT64. */
+int msh_output_kind(const msh_field* field, const msh_params* params, int32_t* kind);
+
+/* Phase 1 of isosurface(): classify every cell, count, and scan the counts
+ * in the reference's emission order (x outermost, z innermost).  Returns the
+ * exact output sizes.  The scan stays in the context for msh_extract. */
+int msh_count(void* ctx, const msh_field* field, const msh_params* params,
+              int64_t* nverts, int64_t* nfaces);
+
+/* Phase 2: write vertices (3*nverts values of msh_output_kind) and faces
+ * (3*nfaces int64, 1-based) for the field/params of the preceding msh_count
+ * on this context.  `out_memory` says where verts/faces live.  Device outputs
+ * are written asynchronously on the context's stream; host outputs are
+ * complete on return. */
+int msh_extract(void* ctx, const msh_field* field, const msh_params* params,
+                void* verts, int64_t* faces, int32_t out_memory);
+
+/* Synthetic fields, generated on the device into a column-major buffer of
+ * nx*ny*nz elements of `dtype`.  kind: 0 sphere sqrt(x^2+y^2+z^2)-p0 on
+ * [-1,1]^3; 1 torus (R=p0, r=p1) on [-1,1]^3; 2 gyroid on [-p0,p0]^3;
+ * 3 value noise (32^3 lattice + 0.25 x 128^3 octave, seeded); 4 sum of 16
+ * seeded Gaussians on [-1,1]^3.  No counterpart in the reference (its tests
+ * build fields with comprehensions, test/runtests.jl:10-13). */
+int msh_generate_field(void* ctx, int32_t kind, int32_t dtype, int64_t nx, int64_t ny,
+                       int64_t nz, uint64_t seed, double p0, double p1, void* out_device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MESHING_HIP_H */

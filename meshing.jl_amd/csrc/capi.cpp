@@ -1,0 +1,419 @@
+// capi.cpp -- the C ABI of libmeshing_hip.so (declared in include/meshing_hip.h).
+//
+// Replaces the reference's `isosurface(sdf, method, X, Y, Z)` specialisations
+// (src/isosurface.jl:30-32, src/marching_cubes.jl:27-64,
+// src/marching_tetrahedra.jl:129-163) with a two-phase call: msh_count
+// (classify + count + scan, returns exact sizes) and msh_extract (emit into
+// caller-owned buffers).  Julia's type promotion is resolved here, on the
+// host, into the handful of arithmetic types the kernels are instantiated
+// for (internal.h, struct Plan).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+
+#include "../../include/meshing_hip.h"
+#include "internal.h"
+
+using namespace msh;
+
+namespace {
+
+struct Ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // grow-only device buffers
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  void* fstage = nullptr;
+  size_t fstage_bytes = 0;
+  void* ostage = nullptr;
+  size_t ostage_bytes = 0;
+  int64_t* pinned = nullptr;  // 2 x int64 for the totals
+  // state of the last successful msh_count
+  bool have = false;
+  msh_field field{};
+  msh_params params{};
+  Plan plan{};
+  Geom geom{};
+  Workspace w{};
+  const void* dfield = nullptr;
+  int64_t nverts = 0, nfaces = 0;
+};
+
+int fail(Ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(Ctx* c, hipError_t e, const char* what) {
+  return fail(c, e == hipErrorOutOfMemory ? MSH_ERR_OUT_OF_MEMORY : MSH_ERR_DEVICE,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+bool valid_kind(int k) { return k == K_INT64 || k == K_F32 || k == K_F64; }
+
+// Smallest float >= the exact value of x (an int64 or a double).
+float ceil_to_float(long double x) {
+  float f = (float)x;
+  if (!isnan(f) && (long double)f < x) f = nextafterf(f, INFINITY);
+  return f;
+}
+double ceil_to_double(long double x) {
+  double d = (double)x;
+  if (!isnan(d) && (long double)d < x) d = nextafter(d, INFINITY);
+  return d;
+}
+
+// Resolve Julia's promotion for this call (src/marching_cubes.jl:31,
+// :100-104; src/marching_tetrahedra.jl:43-54,131) into a Plan.
+int make_plan(Ctx* c, const msh_field* f, const msh_params* p, Plan* out) {
+  if (!f || !p) return fail(c, MSH_ERR_ARGUMENT, "null field or params");
+  if (f->dtype != K_F32 && f->dtype != K_F64)
+    return fail(c, MSH_ERR_UNSUPPORTED, "field element type must be Float32 or Float64");
+  if (f->memory != MSH_HOST && f->memory != MSH_DEVICE)
+    return fail(c, MSH_ERR_ARGUMENT, "field memory must be MSH_HOST or MSH_DEVICE");
+  if (f->nx < 0 || f->ny < 0 || f->nz < 0)
+    return fail(c, MSH_ERR_ARGUMENT, "negative array size");
+  if (f->nx > INT32_MAX / 2 || f->ny > INT32_MAX / 2 || f->nz > INT32_MAX / 2)
+    return fail(c, MSH_ERR_ARGUMENT, "array dimension too large");
+  if (p->method != MSH_MARCHING_CUBES && p->method != MSH_MARCHING_TETRAHEDRA &&
+      p->method != MSH_NAIVE_SURFACE_NETS)
+    return fail(c, MSH_ERR_ARGUMENT, "unknown meshing algorithm");
+  if (!valid_kind(p->iso_kind) || !valid_kind(p->axis_kind[0]) || !valid_kind(p->axis_kind[1]) ||
+      !valid_kind(p->axis_kind[2]) ||
+      (p->method == MSH_MARCHING_TETRAHEDRA && !valid_kind(p->eps_kind)))
+    return fail(c, MSH_ERR_UNSUPPORTED, "iso, eps and axis types must be Int64, Float32 or Float64");
+  if (p->iso_kind == K_INT64 && (p->iso_int > (1ll << 53) || p->iso_int < -(1ll << 53)))
+    return fail(c, MSH_ERR_UNSUPPORTED, "integer iso beyond 2^53 is not supported");
+
+  Plan pl{};
+  pl.method = p->method;
+  pl.T = f->dtype;
+  const int I = p->iso_kind;
+  pl.M = (pl.T == K_F64 || I == K_F64) ? K_F64 : K_F32;
+  const bool mt = p->method == MSH_MARCHING_TETRAHEDRA;
+  const int E = mt ? p->eps_kind : K_INT64;
+  pl.A = (pl.M == K_F64 || E == K_F64) ? K_F64 : K_F32;
+  const bool any64 = pl.T == K_F64 || I == K_F64 || E == K_F64 || p->axis_kind[0] == K_F64 ||
+                     p->axis_kind[1] == K_F64 || p->axis_kind[2] == K_F64;
+  pl.Out = any64 ? K_F64 : K_F32;
+
+  // iso in M, and the classification threshold in T
+  long double iso_exact;
+  if (I == K_INT64) {
+    iso_exact = (long double)p->iso_int;
+    pl.isoM = pl.M == K_F32 ? (double)(float)p->iso_int : (double)p->iso_int;
+  } else if (I == K_F32) {
+    iso_exact = (long double)(float)p->iso;
+    pl.isoM = (double)(float)p->iso;
+  } else {
+    iso_exact = (long double)p->iso;
+    pl.isoM = p->iso;
+  }
+  if (I != K_INT64 && isnan(p->iso)) {
+    pl.thr = NAN;
+  } else {
+    pl.thr = pl.T == K_F32 ? (double)ceil_to_float(iso_exact) : ceil_to_double(iso_exact);
+  }
+
+  // MT: eps in A and one(T) - eps in promote(T, E)
+  if (mt) {
+    if (E == K_INT64) {
+      pl.epsA = pl.A == K_F32 ? (double)(float)p->eps_int : (double)p->eps_int;
+      pl.omeA = pl.T == K_F32 ? (double)(1.0f - (float)p->eps_int) : 1.0 - (double)p->eps_int;
+    } else if (E == K_F32) {
+      const float e = (float)p->eps;
+      pl.epsA = (double)e;
+      pl.omeA = pl.T == K_F32 ? (double)(1.0f - e) : 1.0 - (double)e;
+    } else {
+      pl.epsA = p->eps;
+      pl.omeA = 1.0 - p->eps;
+    }
+  }
+
+  // LinRange(first(X), last(X), n): element type Float32 for Float32 ends,
+  // Float64 otherwise; length 1 requires equal ends (Julia ArgumentError).
+  const int64_t n[3] = {f->nx, f->ny, f->nz};
+  for (int k = 0; k < 3; ++k) {
+    pl.cx[k] = p->axis_kind[k] == K_F32 ? K_F32 : K_F64;
+    if (pl.cx[k] == K_F32) {
+      pl.a[k] = (double)(float)p->first[k];
+      pl.b[k] = (double)(float)p->last[k];
+    } else {
+      pl.a[k] = p->first[k];
+      pl.b[k] = p->last[k];
+    }
+    if (n[k] == 1 && !(pl.a[k] == pl.b[k]))
+      return fail(c, MSH_ERR_ARGUMENT,
+                  "ArgumentError: range(" + std::to_string(pl.a[k]) + ", stop=" +
+                      std::to_string(pl.b[k]) + ", length=1): endpoints differ");
+  }
+  *out = pl;
+  return MSH_OK;
+}
+
+Geom make_geom(const msh_field* f) {
+  Geom g{};
+  g.nx = (int32_t)f->nx;
+  g.ny = (int32_t)f->ny;
+  g.nz = (int32_t)f->nz;
+  const int64_t cx = f->nx - 1, cy = f->ny - 1, cz = f->nz - 1;
+  if (cx <= 0 || cy <= 0 || cz <= 0) {
+    g.zseg = 1;
+    g.nzseg = 0;
+    g.units = 0;
+    return g;
+  }
+  // Split z rows into segments when the xy tiling alone gives too few
+  // blocks to fill 256 CUs; segments of at most 4096 cells keep the packed
+  // 16-bit per-unit counts from overflowing.
+  const int64_t tiles_xy = ((cx + 30) / 31) * ((cy + 3) / 4);
+  int64_t nseg = (2048 + tiles_xy - 1) / tiles_xy;
+  const int64_t max_seg = (cz + 63) / 64;
+  if (nseg > max_seg) nseg = max_seg;
+  if (nseg < 1) nseg = 1;
+  const int64_t min_seg = (cz + 4095) / 4096;
+  if (nseg < min_seg) nseg = min_seg;
+  int64_t zseg = (cz + nseg - 1) / nseg;
+  zseg = (zseg + 63) / 64 * 64;
+  if (zseg > 4096) zseg = 4096;
+  nseg = (cz + zseg - 1) / zseg;
+  g.zseg = (int32_t)zseg;
+  g.nzseg = (int32_t)nseg;
+  g.units = cx * cy * nseg;
+  return g;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int grow(Ctx* c, void** buf, size_t* have, size_t need, const char* what) {
+  if (need <= *have) return MSH_OK;
+  if (*buf) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+  }
+  hipError_t e = hipMalloc(buf, need);
+  if (e != hipSuccess) {
+    *buf = nullptr;
+    return hip_fail(c, e, what);
+  }
+  *have = need;
+  return MSH_OK;
+}
+
+int setup_workspace(Ctx* c, const Geom& g, Workspace* w) {
+  const int64_t U = g.units > 0 ? g.units : 1;
+  const int64_t nt = scan_tiles(U);
+  const size_t b_cnt = align256((size_t)U * 4);
+  const size_t b_off = align256((size_t)U * 8);
+  const size_t b_status = align256((size_t)(nt + 1) * 8);
+  const size_t b_incl = align256((size_t)nt * 16);
+  const size_t b_tot = 256;
+  const size_t b_coords = align256((size_t)(g.nx + g.ny + g.nz + 1) * 8);
+  const size_t need = b_cnt + 2 * b_off + b_status + b_incl + b_tot + b_coords;
+  int rc = grow(c, &c->ws, &c->ws_bytes, need, "workspace allocation");
+  if (rc) return rc;
+  char* p = (char*)c->ws;
+  w->cnt = (uint32_t*)p;
+  p += b_cnt;
+  w->voff = (int64_t*)p;
+  p += b_off;
+  w->foff = (int64_t*)p;
+  p += b_off;
+  w->status = (uint64_t*)p;
+  w->counter = (uint32_t*)(w->status + nt);
+  p += b_status;
+  w->incl = (int64_t*)p;
+  p += b_incl;
+  w->totals = (int64_t*)p;
+  p += b_tot;
+  w->coords = (double*)p;
+  return MSH_OK;
+}
+
+bool same_call(const Ctx* c, const msh_field* f, const msh_params* p) {
+  return c->have && memcmp(&c->field, f, sizeof(*f)) == 0 &&
+         memcmp(&c->params, p, sizeof(*p)) == 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int msh_create(int device, void** out) {
+  if (!out) return MSH_ERR_ARGUMENT;
+  *out = nullptr;
+  Ctx* c = new (std::nothrow) Ctx();
+  if (!c) return MSH_ERR_OUT_OF_MEMORY;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->pinned, 2 * sizeof(int64_t), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+    return e == hipErrorOutOfMemory ? MSH_ERR_OUT_OF_MEMORY : MSH_ERR_DEVICE;
+  }
+  c->device = device;
+  c->stream = c->own;
+  *out = c;
+  return MSH_OK;
+}
+
+int msh_destroy(void* ctx) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->ws) (void)hipFree(c->ws);
+  if (c->fstage) (void)hipFree(c->fstage);
+  if (c->ostage) (void)hipFree(c->ostage);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+  return MSH_OK;
+}
+
+int msh_set_stream(void* ctx, void* s) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  // NULL is HIP's default (null) stream, which is what torch reports for
+  // its default stream; it is used as given, never replaced by our own.
+  c->stream = (hipStream_t)s;
+  return MSH_OK;
+}
+
+int msh_synchronize(void* ctx) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? MSH_OK : hip_fail(c, e, "hipStreamSynchronize");
+}
+
+const char* msh_last_error(const void* ctx) {
+  const Ctx* c = (const Ctx*)ctx;
+  return c ? c->err.c_str() : "null context";
+}
+
+int msh_output_kind(const msh_field* f, const msh_params* p, int32_t* kind) {
+  Plan pl;
+  msh_field f1 = *f;
+  // the kind does not depend on the sizes; avoid the length-1 range check
+  f1.nx = f1.ny = f1.nz = 2;
+  int rc = make_plan(nullptr, &f1, p, &pl);
+  if (rc) return rc;
+  *kind = pl.Out;
+  return MSH_OK;
+}
+
+int msh_count(void* ctx, const msh_field* f, const msh_params* p, int64_t* nverts,
+              int64_t* nfaces) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  c->have = false;
+  Plan pl;
+  int rc = make_plan(c, f, p, &pl);
+  if (rc) return rc;
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
+  Geom g = make_geom(f);
+  Workspace w{};
+  rc = setup_workspace(c, g, &w);
+  if (rc) return rc;
+  int64_t nv = 0, nfc = 0;
+  const void* dfield = f->data;
+  if (g.units > 0) {
+    const size_t esz = f->dtype == K_F32 ? 4 : 8;
+    const size_t fbytes = (size_t)f->nx * f->ny * f->nz * esz;
+    if (!f->data) return fail(c, MSH_ERR_ARGUMENT, "null field data");
+    if (f->memory == MSH_HOST) {
+      rc = grow(c, &c->fstage, &c->fstage_bytes, fbytes, "field staging allocation");
+      if (rc) return rc;
+      hipError_t e = hipMemcpyAsync(c->fstage, f->data, fbytes, hipMemcpyHostToDevice, c->stream);
+      if (e != hipSuccess) return hip_fail(c, e, "field upload");
+      dfield = c->fstage;
+    }
+    hipError_t e = launch_coords(pl, g, w.coords, c->stream);
+    if (e == hipSuccess) e = launch_count(pl, g, dfield, w.cnt, c->stream);
+    if (e == hipSuccess) e = launch_scan(g, w, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->pinned, w.totals, 2 * sizeof(int64_t), hipMemcpyDeviceToHost,
+                         c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "count/scan");
+    nv = c->pinned[0];
+    nfc = c->pinned[1];
+  }
+  c->field = *f;
+  c->params = *p;
+  c->plan = pl;
+  c->geom = g;
+  c->w = w;
+  c->dfield = dfield;
+  c->nverts = nv;
+  c->nfaces = nfc;
+  c->have = true;
+  if (nverts) *nverts = nv;
+  if (nfaces) *nfaces = nfc;
+  return MSH_OK;
+}
+
+int msh_extract(void* ctx, const msh_field* f, const msh_params* p, void* verts,
+                int64_t* faces, int32_t out_memory) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  if (!f || !p) return fail(c, MSH_ERR_ARGUMENT, "null field or params");
+  if (!same_call(c, f, p))
+    return fail(c, MSH_ERR_STATE, "msh_extract must follow msh_count with the same arguments");
+  if (out_memory != MSH_HOST && out_memory != MSH_DEVICE)
+    return fail(c, MSH_ERR_ARGUMENT, "out_memory must be MSH_HOST or MSH_DEVICE");
+  if (c->nverts == 0 && c->nfaces == 0) return MSH_OK;
+  if ((c->nverts > 0 && !verts) || (c->nfaces > 0 && !faces))
+    return fail(c, MSH_ERR_ARGUMENT, "null output buffer");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
+  const size_t osz = c->plan.Out == K_F32 ? 4 : 8;
+  const size_t vbytes = (size_t)c->nverts * 3 * osz, fbytes = (size_t)c->nfaces * 3 * 8;
+  void* dv = verts;
+  int64_t* df = faces;
+  if (out_memory == MSH_HOST) {
+    int rc = grow(c, &c->ostage, &c->ostage_bytes, align256(vbytes) + fbytes + 256,
+                  "output staging allocation");
+    if (rc) return rc;
+    dv = c->ostage;
+    df = (int64_t*)((char*)c->ostage + align256(vbytes));
+  }
+  hipError_t e = launch_emit(c->plan, c->geom, c->dfield, c->w, dv, df, c->stream);
+  if (e == hipErrorNotSupported)
+    return fail(c, MSH_ERR_UNSUPPORTED, "this method is not available on the GPU path yet");
+  if (e != hipSuccess) return hip_fail(c, e, "emit");
+  if (out_memory == MSH_HOST) {
+    if (vbytes) e = hipMemcpyAsync(verts, dv, vbytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && fbytes)
+      e = hipMemcpyAsync(faces, df, fbytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "output download");
+  }
+  return MSH_OK;
+}
+
+int msh_generate_field(void* ctx, int32_t kind, int32_t dtype, int64_t nx, int64_t ny,
+                       int64_t nz, uint64_t seed, double p0, double p1, void* out) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  if (kind < 0 || kind > 4) return fail(c, MSH_ERR_ARGUMENT, "unknown field kind");
+  if (dtype != K_F32 && dtype != K_F64) return fail(c, MSH_ERR_UNSUPPORTED, "dtype");
+  if (nx < 0 || ny < 0 || nz < 0) return fail(c, MSH_ERR_ARGUMENT, "negative size");
+  if (nx * ny * nz > 0 && !out) return fail(c, MSH_ERR_ARGUMENT, "null output");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
+  hipError_t e = launch_generate(kind, dtype, nx, ny, nz, seed, p0, p1, out, c->stream);
+  return e == hipSuccess ? MSH_OK : hip_fail(c, e, "generate");
+}
+
+}  // extern "C"

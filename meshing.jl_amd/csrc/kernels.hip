@@ -1,0 +1,1099 @@
+// kernels.hip -- gfx950 kernels of the isosurface path.
+//
+// Pipeline per extraction (all on one stream):
+//   k_coords   LinRange coordinates of the three axes (Julia lerpi).
+//   k_count    classify every cell and count, per z-row segment ("unit"),
+//              the vertices and faces it will emit.  Lanes run along x
+//              (coalesced HBM reads); each lane marches z, keeping the
+//              previous sample plane in registers and taking the x+1
+//              neighbour from the next lane.
+//   k_scan     single-pass decoupled look-back scan of the unit counts in
+//              the reference's emission order (x outermost, z innermost).
+//   k_emit_*   re-read the field brick by brick into LDS, run each z-row
+//              chunk of 64 cells as one wavefront (lanes along z, the
+//              emission order), wave-scan the per-cell counts and write the
+//              chunk's vertices and faces slot-parallel: lane s writes
+//              output s, so stores are contiguous.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "internal.h"
+
+#define MSH_LUT static __constant__
+#include "luts.h"
+
+namespace msh {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / kWave;
+
+// Corner offsets.  Marching Cubes order (src/marching_cubes.jl:42-49):
+// (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1).
+__device__ __forceinline__ int mc_dx(int c) { return (0x66 >> c) & 1; }
+__device__ __forceinline__ int mc_dy(int c) { return (0xCC >> c) & 1; }
+__device__ __forceinline__ int mc_dz(int c) { return (0xF0 >> c) & 1; }
+// Marching Tetrahedra order (src/marching_tetrahedra.jl:146-153):
+// (0,0,0) (0,1,0) (1,1,0) (1,0,0) (0,0,1) (0,1,1) (1,1,1) (1,0,1).
+__device__ __forceinline__ int mt_dx(int c) { return (0xCC >> c) & 1; }
+__device__ __forceinline__ int mt_dy(int c) { return (0x66 >> c) & 1; }
+__device__ __forceinline__ int mt_dz(int c) { return (0xF0 >> c) & 1; }
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Make this wave's LDS writes visible to its own later LDS reads.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint64_t t = __shfl_up(v, d, kWave);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Workgroup barrier ordering LDS only: it does not wait for this wave's
+// outstanding global stores, which __syncthreads() would.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Lane i gets lane i+1's value (DPP wave_shl:1); lane 63 gets `last`.
+__device__ __forceinline__ uint32_t next_lane(uint32_t v, uint32_t last) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)v, 0x130, 0xF, 0xF, false);
+}
+
+// Blocks of one launch are dealt round-robin to the 8 XCDs (each with its
+// own L2).  Renumber so that each XCD gets a contiguous range of tiles:
+// neighbouring tiles (which share halo samples) then meet in the same L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+  const int64_t xcd = b & 7, local = b >> 3;
+  const int64_t per = nb >> 3, rem = nb & 7;
+  return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
+// ---------------------------------------------------------------------------
+// LinRange coordinates: xp[i] = CX((1-t)*a + t*b), t = (i-1)/max(1, n-1)
+// (Julia Base lerpi; used at src/marching_cubes.jl:36-38).
+__global__ void k_coords(double* __restrict__ out, int nx, int ny, int nz, double ax, double bx,
+                         double ay, double by, double az, double bz, int cxmask) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int axis, n, j;
+  if (i < nx) {
+    axis = 0; n = nx; j = i;
+  } else if (i < nx + ny) {
+    axis = 1; n = ny; j = i - nx;
+  } else if (i < nx + ny + nz) {
+    axis = 2; n = nz; j = i - nx - ny;
+  } else {
+    return;
+  }
+  const double a = axis == 0 ? ax : axis == 1 ? ay : az;
+  const double b = axis == 0 ? bx : axis == 1 ? by : bz;
+  const int d = n - 1 > 1 ? n - 1 : 1;
+  const double t = (double)j / (double)d;
+  double v = (1.0 - t) * a + t * b;
+  if ((cxmask >> axis) & 1) v = (double)(float)v;
+  out[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Count.  Block = 4 waves; wave w covers sample columns x in [xb, xb+64)
+// (cells xb..xb+62) and TY rows of cells starting at yb; it marches the z
+// segment of blockIdx.z.  Per plane it loads TY+1 samples per lane, turns
+// them into a bit mask (bit r = sample row r is below iso), and gets the
+// x+1 column's mask from the next lane.
+template <int METHOD>
+__device__ __forceinline__ uint32_t plane_code(uint32_t m, uint32_t mn, int r) {
+  const uint32_t b00 = (m >> r) & 1, b01 = (m >> (r + 1)) & 1;   // (x, y), (x, y+1)
+  const uint32_t b10 = (mn >> r) & 1, b11 = (mn >> (r + 1)) & 1; // (x+1, y), (x+1, y+1)
+  if (METHOD == MT) return b00 | (b01 << 1) | (b11 << 2) | (b10 << 3);
+  return b00 | (b10 << 1) | (b11 << 2) | (b01 << 3);
+}
+
+template <typename T, int METHOD, int TY, int ZU>
+__global__ __launch_bounds__(kBlock) void k_count(const T* __restrict__ f, Geom g, T thr,
+                                                  uint32_t* __restrict__ cnt, int nbx, int nby,
+                                                  int64_t nblocks) {
+  __shared__ uint16_t lut[METHOD == MT ? 2048 : 256];
+  if (METHOD == MC) {
+    for (int c = threadIdx.x; c < 256; c += kBlock)
+      lut[c] = (uint16_t)((MC_CASE_VERTS[c] & 0xF) | ((MC_CASE_TRIS[c] & 0x7) << 8));
+  } else if (METHOD == MT) {
+    for (int c = threadIdx.x; c < 2048; c += kBlock)
+      lut[c] = (uint16_t)(MT_NOWN[c] | (MT_NF[c & 255] << 8));
+  }
+  __syncthreads();
+
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  // y-neighbouring tiles (which share a halo row of samples) run on the
+  // same XCD, so the shared row is read from HBM once.
+  const int64_t t = xcd_tile(blockIdx.x, nblocks);
+  const int by = (int)(t % nby);
+  const int64_t rest = t / nby;
+  const int bx = (int)(rest % nbx);
+  const int s = (int)(rest / nbx);
+  const int xb = bx * (kWave - 1);
+  const int yb = (by * kWaves + wv) * TY;
+  if (yb >= g.ny - 1) return;
+  const int zs = s * g.zseg;
+  const int ze = min(zs + g.zseg, g.nz - 1);
+  const int xs = min(xb + lane, g.nx - 1);
+  const bool xvalid = lane < kWave - 1 && xb + lane < g.nx - 1;
+  const int64_t pstride = (int64_t)g.nx * g.ny;
+
+  int64_t roff[TY + 1];
+#pragma unroll
+  for (int r = 0; r <= TY; ++r) roff[r] = (int64_t)min(yb + r, g.ny - 1) * g.nx + xs;
+
+  auto mask_of = [&](int kk) {
+    const T* p = f + (int64_t)kk * pstride;
+    uint32_t m = 0;
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) m |= (uint32_t)(p[roff[r]] < thr) << r;
+    return m;
+  };
+
+  uint32_t acc[TY];
+#pragma unroll
+  for (int r = 0; r < TY; ++r) acc[r] = 0;
+
+  uint32_t m0 = mask_of(zs);
+  uint32_t m0n = next_lane(m0, 0u);
+  for (int k = zs; k < ze; k += ZU) {
+    T v[ZU][TY + 1];
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      const T* p = f + (int64_t)min(k + 1 + u, g.nz - 1) * pstride;
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) v[u][r] = p[roff[r]];
+    }
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      if (k + u >= ze) break;
+      uint32_t m1 = 0;
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) m1 |= (uint32_t)(v[u][r] < thr) << r;
+      const uint32_t m1n = next_lane(m1, 0u);
+      const int kc = k + u;
+#pragma unroll
+      for (int r = 0; r < TY; ++r) {
+        const uint32_t cs = plane_code<METHOD>(m0, m0n, r) | (plane_code<METHOD>(m1, m1n, r) << 4);
+        if (METHOD == MC) {
+          const uint32_t e = lut[cs];
+          acc[r] += (e & 0xF) | ((e >> 8) << 16);
+        } else if (METHOD == MT) {
+          const int cls = (xb + lane == 0) | ((yb + r == 0) << 1) | ((kc == 0) << 2);
+          const uint32_t e = lut[cls * 256 + cs];
+          acc[r] += (e & 0xFF) | ((e >> 8) << 16);
+        } else {
+          if (cs != 0 && cs != 255) {
+            const uint32_t c0 = cs & 1;
+            const int i = xb + lane, j = yb + r;
+            uint32_t nf = 0;
+            nf += (c0 != ((cs >> 1) & 1) && j >= 1 && kc >= 1) ? 2 : 0;  // x edge
+            nf += (c0 != ((cs >> 3) & 1) && kc >= 1 && i >= 1) ? 2 : 0;  // y edge
+            nf += (c0 != ((cs >> 4) & 1) && i >= 1 && j >= 1) ? 2 : 0;   // z edge
+            acc[r] += 1 | (nf << 16);
+          }
+        }
+      }
+      m0 = m1;
+      m0n = m1n;
+    }
+  }
+  if (!xvalid) return;
+#pragma unroll
+  for (int r = 0; r < TY; ++r) {
+    const int j = yb + r;
+    if (j < g.ny - 1)
+      cnt[((int64_t)(xb + lane) * (g.ny - 1) + j) * g.nzseg + s] = acc[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scan: exclusive prefix of the unit counts, single pass with decoupled
+// look-back.  status[t]: bits 62-63 flag (1 aggregate, 2 inclusive),
+// bits 0-30 tile vertex count, bits 31-61 tile face count.  incl[2t..2t+1]
+// hold the inclusive prefixes, published before the flag turns 2.
+__device__ __forceinline__ uint64_t ld_acquire(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int64_t ld_relaxed(const int64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(int64_t* p, int64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_zero_u64(uint64_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ cnt, int64_t units,
+                                                 int64_t* __restrict__ voff,
+                                                 int64_t* __restrict__ foff,
+                                                 uint64_t* __restrict__ status,
+                                                 int64_t* __restrict__ incl,
+                                                 uint32_t* __restrict__ counter,
+                                                 int64_t* __restrict__ totals, int64_t ntiles) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_wsum[kWaves];
+  __shared__ int64_t s_excl[2];
+  if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  const int64_t base = tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
+
+  uint32_t c[kScanItems];
+  uint64_t sum = 0;  // vertices in the low 32 bits, faces in the high 32 bits
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    c[q] = base + q < units ? cnt[base + q] : 0u;
+    sum += (uint64_t)(c[q] & 0xFFFF) | ((uint64_t)(c[q] >> 16) << 32);
+  }
+  const uint64_t wincl = wave_incl_scan64(sum);
+  if (lane == kWave - 1) s_wsum[wv] = wincl;
+  __syncthreads();
+  uint64_t before = 0, agg = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) {
+    if (q < wv) before += s_wsum[q];
+    agg += s_wsum[q];
+  }
+  const uint64_t texcl = before + wincl - sum;
+  const int64_t agg_v = (int64_t)(agg & 0xFFFFFFFFu), agg_f = (int64_t)(agg >> 32);
+
+  if (wv == 0) {
+    int64_t ev = 0, ef = 0;
+    if (tile == 0) {
+      if (lane == 0) {
+        st_relaxed(&incl[0], agg_v);
+        st_relaxed(&incl[1], agg_f);
+        st_release(&status[0], 2ull << 62);
+      }
+    } else {
+      if (lane == 0)
+        st_release(&status[tile], (1ull << 62) | (uint64_t)agg_v | ((uint64_t)agg_f << 31));
+      int64_t j = tile - 1;
+      while (true) {
+        const int64_t t = j - lane;
+        uint64_t st = 0;
+        if (t >= 0) {
+          do {
+            st = ld_acquire(&status[t]);
+          } while ((st >> 62) == 0);
+        }
+        const bool incl_here = t >= 0 && (st >> 62) == 2;
+        const uint64_t ball = __ballot(incl_here);
+        int64_t cv = 0, cf = 0;
+        if (ball) {
+          const int first = __ffsll((unsigned long long)ball) - 1;
+          if (lane < first) {
+            cv = (int64_t)(st & 0x7FFFFFFFull);
+            cf = (int64_t)((st >> 31) & 0x7FFFFFFFull);
+          } else if (lane == first) {
+            cv = ld_relaxed(&incl[2 * t]);
+            cf = ld_relaxed(&incl[2 * t + 1]);
+          }
+        } else if (t >= 0) {
+          cv = (int64_t)(st & 0x7FFFFFFFull);
+          cf = (int64_t)((st >> 31) & 0x7FFFFFFFull);
+        }
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) {
+          cv += __shfl_xor(cv, d, kWave);
+          cf += __shfl_xor(cf, d, kWave);
+        }
+        ev += cv;
+        ef += cf;
+        if (ball) break;
+        j -= kWave;
+      }
+      if (lane == 0) {
+        st_relaxed(&incl[2 * tile], ev + agg_v);
+        st_relaxed(&incl[2 * tile + 1], ef + agg_f);
+        st_release(&status[tile], 2ull << 62);
+      }
+    }
+    if (lane == 0) {
+      s_excl[0] = ev;
+      s_excl[1] = ef;
+      if (tile == ntiles - 1) {
+        totals[0] = ev + agg_v;
+        totals[1] = ef + agg_f;
+      }
+    }
+  }
+  __syncthreads();
+  int64_t rv = s_excl[0] + (int64_t)(texcl & 0xFFFFFFFFu);
+  int64_t rf = s_excl[1] + (int64_t)(texcl >> 32);
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    if (base + q < units) {
+      voff[base + q] = rv;
+      foff[base + q] = rf;
+    }
+    rv += c[q] & 0xFFFF;
+    rf += c[q] >> 16;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Emit.
+//
+// One block = a brick of TX x TY cell rows (plus, for MT and SN, a one-cell
+// halo on the low x and y sides); it walks the z segment of its tile in
+// chunks of 64 cells.  Per chunk the samples of the brick are staged in LDS
+// (plane stride padded to an odd number of words so lanes reading along z
+// hit distinct banks); the next chunk's samples are already being fetched
+// into registers while this one is processed.  Each wave takes cell rows
+// (i, j): lane l is cell (i, j, zc+l), so the lanes follow the reference's
+// emission order, one wave scan gives every cell's output slot, and the
+// chunk's outputs of a row form one contiguous range per output array.
+// Outputs are written slot-parallel (lane s writes output s of the row).
+//
+// MT and SN faces refer to vertices created by the neighbouring cells at
+// -x, -y, -z.  Those cells' cases and in-row vertex prefixes are in LDS
+// (halo rows, and the plane below the chunk), so a face's vertex id is
+// (row base + prefix of the owning cell + rank of the edge in that cell).
+struct EmitArgs {
+  const void* f;
+  Geom g;
+  double thr, iso;         // threshold in T, iso in M
+  double eps, ome;         // MT: eps and one(T)-eps, in A
+  int cxmask;              // bit k: axis k coordinates are Float32
+  const double* coords;    // xp | yp | zp
+  const int64_t* voff;
+  const int64_t* foff;
+  const int64_t* totals;   // (nverts, nfaces): writes beyond are dropped
+  int nti, ntj;            // tiles along x and y
+  int64_t ntiles;
+};
+
+// Inclusive scan over the 64 lanes with DPP: Kogge-Stone within each row of
+// 16 lanes, then row broadcasts of lanes 15 and 31.
+__device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return v;
+}
+
+// One axis of p1 .+ mu .* (p2 .- p1) (src/marching_cubes.jl:100-104) with
+// p1, p2 of the axis element type; the result type is promote(M, CX).
+template <typename M>
+__device__ __forceinline__ double lerp_axis(bool cx32, M mu, double p1, double p2) {
+  if (cx32) {
+    const float a = (float)p1, b = (float)p2;
+    return (double)(a + mu * (b - a));
+  }
+  return (double)(p1 + mu * (p2 - p1));
+}
+
+// One axis of vertPos: base .+ (c1*b .+ c2*a) .* (next - base)
+// (src/marching_tetrahedra.jl:49-54), with w = c1*b + c2*a of type A.
+template <typename A>
+__device__ __forceinline__ double mt_axis(bool cx32, A w, double lo, double hi) {
+  if (cx32) {
+    const float l = (float)lo, h = (float)hi;
+    const float d = h - l;
+    return (double)(l + w * d);
+  }
+  const double d = hi - lo;
+  return (double)(lo + w * d);
+}
+
+// Julia's min/max of floats: NaN-propagating, -0.0 < +0.0.
+template <typename C>
+__device__ __forceinline__ C jl_min(C x, C y) {
+  if (isnan(x)) return x;
+  if (isnan(y)) return y;
+  return signbit(x - y) ? x : y;
+}
+template <typename C>
+__device__ __forceinline__ C jl_max(C x, C y) {
+  if (isnan(x)) return x;
+  if (isnan(y)) return y;
+  return signbit(x - y) ? y : x;
+}
+
+template <int METHOD, typename T, int TX, int TY>
+struct EmitCfg {
+  static constexpr int H = METHOD == MC ? 0 : 1;       // low-side halo
+  static constexpr int RX = TX + H, RY = TY + H;       // cell rows incl. halo
+  static constexpr int NR = RX * RY;
+  static constexpr int SX = RX + 1, SY = RY + 1;       // samples per plane
+  static constexpr int NP = kWave + 1 + H;             // sample planes per chunk
+  static constexpr int PS = (SX * SY) | 1;             // padded plane stride
+  static constexpr int CAP = 256;                      // active-cell list per wave
+};
+
+template <int METHOD, typename T, int TX, int TY>
+struct EmitSmem {
+  using C = EmitCfg<METHOD, T, TX, TY>;
+  T smp[2][C::PS * C::NP];          // double-buffered brick, filled by LDS DMA
+  double cz[2][C::NP];
+  double cx[C::SX], cy[C::SY];
+  int64_t rbv[2][C::NR], rbf[2][C::NR];  // offsets of each row's first cell: this/next chunk
+  uint8_t ccase[C::H ? C::NR * kWave : 1];
+  uint16_t cpv[C::H ? C::NR * kWave : 1];
+  uint8_t pcase[C::H ? C::NR : 1];  // case of the cell just below the chunk
+  uint64_t lut_v[METHOD == MC ? 256 : 1];  // MC_CASE_VERTS
+  uint64_t lut_t[METHOD == MC ? 256 : 1];  // MC_CASE_TRIS
+  uint8_t lut_e[16];                       // MC_EDGE
+  uint64_t list[kWaves][C::CAP];    // active cells: row | z<<8 | case<<14 | pv<<22 | pf<<32
+};
+
+// 4-bit code of the sample plane under a cell's lower face, in the corner
+// order of METHOD.  p points at sample (ci, cj) of that plane.
+template <int METHOD, typename T, int SX>
+__device__ __forceinline__ uint32_t face_code(const T* p, T thr) {
+  const uint32_t b00 = p[0] < thr, b10 = p[1] < thr, b11 = p[SX + 1] < thr, b01 = p[SX] < thr;
+  if (METHOD == MT) return b00 | (b01 << 1) | (b11 << 2) | (b10 << 3);
+  return b00 | (b10 << 1) | (b11 << 2) | (b01 << 3);
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out, int TX, int TY>
+__global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, Out* __restrict__ verts,
+                                                 int64_t* __restrict__ faces) {
+  using C = EmitCfg<METHOD, T, TX, TY>;
+  constexpr int H = C::H, RX = C::RX, NR = C::NR, SX = C::SX, SY = C::SY, NP = C::NP,
+                PS = C::PS, CAP = C::CAP;
+  __shared__ EmitSmem<METHOD, T, TX, TY> sm;
+  const T* __restrict__ f = (const T*)a.f;
+  const Geom g = a.g;
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+
+  const int64_t t = xcd_tile(blockIdx.x, a.ntiles);
+  const int tj = (int)(t % a.ntj);
+  const int64_t rest = t / a.ntj;
+  const int ti = (int)(rest % a.nti);
+  const int s = (int)(rest / a.nti);
+  const int x0 = ti * TX - H, y0 = tj * TY - H;  // global cell index of brick row 0
+  const int zs = s * g.zseg, ze = min(zs + g.zseg, g.nz - 1);
+  const T thr = (T)a.thr;
+  const M iso = (M)a.iso;
+  const bool cx32 = a.cxmask & 1, cy32 = (a.cxmask >> 1) & 1, cz32 = (a.cxmask >> 2) & 1;
+  const double* xp = a.coords;
+  const double* yp = a.coords + g.nx;
+  const double* zp = a.coords + g.nx + g.ny;
+  const int64_t totv = a.totals[0], totf = a.totals[1];
+  const int64_t pstride = (int64_t)g.nx * g.ny;
+
+  if constexpr (METHOD == MC) {
+    for (int q = threadIdx.x; q < 256; q += kBlock) {
+      sm.lut_v[q] = MC_CASE_VERTS[q];
+      sm.lut_t[q] = MC_CASE_TRIS[q];
+    }
+  }
+  if (METHOD != MT && threadIdx.x < 12) sm.lut_e[threadIdx.x] = MC_EDGE[threadIdx.x];
+  for (int r = threadIdx.x; r < NR; r += kBlock) {
+    const int i = x0 + r % RX, j = y0 + r / RX;
+    int64_t bv = 0, bf = 0;
+    if (i >= 0 && j >= 0 && i < g.nx - 1 && j < g.ny - 1) {
+      const int64_t u = ((int64_t)i * (g.ny - 1) + j) * g.nzseg + s;
+      bv = a.voff[u];
+      bf = a.foff[u];
+    }
+    sm.rbv[0][r] = bv;
+    sm.rbf[0][r] = bf;
+  }
+  for (int q = threadIdx.x; q < SX; q += kBlock) sm.cx[q] = xp[min(max(x0 + q, 0), g.nx - 1)];
+  for (int q = threadIdx.x; q < SY; q += kBlock) sm.cy[q] = yp[min(max(y0 + q, 0), g.ny - 1)];
+
+  // DMA of the brick for the chunk starting at cell zc (planes zc-H ..
+  // zc+64) straight into LDS buffer `buf`: each wave instruction fills 64
+  // consecutive dwords; lanes landing on plane padding load a harmless
+  // in-bounds dword.  No registers hold the data, so the next chunk's
+  // loads stay in flight while this chunk is processed.
+  auto fetch = [&](int zc, int buf) {
+    constexpr int DW = (int)(sizeof(T) / 4);
+    constexpr int NDW = PS * NP * DW;
+    constexpr int NI = (NDW + kWave - 1) / kWave;
+    uint32_t* dst = (uint32_t*)sm.smp[buf];
+#pragma unroll 4
+    for (int ins = wv; ins < NI; ins += kWaves) {
+      const int p = ins * kWave + lane;
+      if (p < NDW) {
+        const int e = p / DW, half = p - e * DW;
+        const int z = e / PS, rem = e - z * PS;
+        const char* src = (const char*)f;
+        if (rem < SX * SY) {
+          const int y = rem / SX, x = rem - y * SX;
+          const int gx = min(max(x0 + x, 0), g.nx - 1), gy = min(max(y0 + y, 0), g.ny - 1),
+                    gz = min(max(zc - H + z, 0), g.nz - 1);
+          src = (const char*)(f + ((int64_t)gz * pstride + (int64_t)gy * g.nx + gx)) + 4 * half;
+        }
+        __builtin_amdgcn_global_load_lds((const void*)src, (void*)(dst + ins * kWave), 4, 0, 0);
+      }
+    }
+    constexpr int NZ = (NP * 2 + kWave - 1) / kWave;  // cz: NP doubles
+    uint32_t* zd = (uint32_t*)sm.cz[buf];
+    for (int ins = wv; ins < NZ; ins += kWaves) {
+      const int p = ins * kWave + lane;
+      if (p < NP * 2) {
+        const int e = p >> 1, half = p & 1;
+        const double* src = zp + min(max(zc - H + e, 0), g.nz - 1);
+        __builtin_amdgcn_global_load_lds((const void*)((const char*)src + 4 * half),
+                                         (void*)(zd + ins * kWave), 4, 0, 0);
+      }
+    }
+  };
+
+  uint64_t* list = sm.list[wv];
+
+  if (zs < ze) fetch(zs, 0);
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's share of the first brick landed
+  lds_barrier();
+  int buf = 0;
+  for (int zc = zs; zc < ze; zc += kWave, buf ^= 1) {
+    // Buffer buf^1 was last read in the previous chunk, which every wave
+    // has left (barrier at its end): refill it for the next chunk now.
+    if (zc + kWave < ze) fetch(zc + kWave, buf ^ 1);
+    const T* smp = sm.smp[buf];
+    const double* czs = sm.cz[buf];
+    const int64_t* rbv = sm.rbv[buf];
+    const int64_t* rbf = sm.rbf[buf];
+    int64_t* rbv_next = sm.rbv[buf ^ 1];
+    int64_t* rbf_next = sm.rbf[buf ^ 1];
+    const int kc = zc + lane;  // this lane's cell z
+    const bool zvalid = kc < ze;
+
+    if constexpr (H) {
+      // Phase 1: classify every row of the brick, halo rows included, and
+      // keep case + in-row vertex prefix for the neighbour lookups.
+      for (int r = wv; r < NR; r += kWaves) {
+        const int ci = r % RX, cj = r / RX;
+        const int gi = x0 + ci, gj = y0 + cj;
+        uint32_t cs = 0, pc = 0, nv = 0;
+        if (gi >= 0 && gj >= 0 && gi < g.nx - 1 && gj < g.ny - 1) {
+          const T* p = smp + cj * SX + ci;
+          const uint32_t lo = face_code<METHOD, T, SX>(p + (lane + 1) * PS, thr);
+          const uint32_t top = face_code<METHOD, T, SX>(p + (kWave + 1) * PS, thr);
+          const uint32_t below = face_code<METHOD, T, SX>(p, thr);
+          const uint32_t hi = next_lane(lo, top);
+          cs = zvalid ? (lo | (hi << 4)) : 0u;
+          const uint32_t lo0 = (uint32_t)__builtin_amdgcn_readlane((int)lo, 0);
+          pc = zc > 0 ? (below | (lo0 << 4)) : 0u;
+          if (METHOD == MT) {
+            const int cls = (gi == 0) | ((gj == 0) << 1) | ((kc == 0) << 2);
+            nv = MT_NOWN[cls * 256 + cs];
+          } else {
+            nv = (cs != 0 && cs != 255) ? 1u : 0u;
+          }
+        }
+        const uint32_t inc = wave_scan_dpp(nv);
+        sm.ccase[r * kWave + lane] = (uint8_t)cs;
+        sm.cpv[r * kWave + lane] = (uint16_t)(inc - nv);
+        if (lane == 0) {
+          sm.pcase[r] = (uint8_t)pc;
+          rbv_next[r] = rbv[r] + (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+        }
+      }
+      lds_barrier();
+    }
+
+    // Emission of one active cell by one lane.  Entry: row | z<<8 |
+    // case<<14 | in-row vertex prefix<<22 | in-row face prefix<<32.
+    auto emit_cell = [&](uint64_t ent) {
+      const int r = (int)(ent & 0xFF), c = (int)((ent >> 8) & 63);
+      const uint32_t cs = (uint32_t)((ent >> 14) & 0xFF);
+      const uint32_t pv = (uint32_t)((ent >> 22) & 0x3FF), pf = (uint32_t)((ent >> 32) & 0x3FF);
+      const int ci = r % RX, cj = r / RX;
+      const int gi = x0 + ci, gj = y0 + cj, gk = zc + c;
+      const int64_t vb = rbv[r] + pv;  // this cell's first vertex (0-based)
+      const int64_t fb = rbf[r] + pf;  // this cell's first face
+      const T* pc = smp + (c + H) * PS + cj * SX + ci;  // corner (0,0,0) of the cell
+      if constexpr (METHOD == MC) {
+        const uint64_t vw = sm.lut_v[cs];
+        const int nv = (int)(vw & 0xF);
+        for (int q = 0; q < nv; ++q) {
+          const uint32_t e = (uint32_t)(vw >> (4 + 4 * q)) & 0xF;
+          const uint32_t ew = sm.lut_e[e];
+          const int ca = ew & 0xF, cb = ew >> 4;
+          const T va = pc[mc_dz(ca) * PS + mc_dy(ca) * SX + mc_dx(ca)];
+          const T vbv = pc[mc_dz(cb) * PS + mc_dy(cb) * SX + mc_dx(cb)];
+          const M mu = (iso - (M)va) / (M)(vbv - va);
+          const double px = lerp_axis<M>(cx32, mu, sm.cx[ci + mc_dx(ca)], sm.cx[ci + mc_dx(cb)]);
+          const double py = lerp_axis<M>(cy32, mu, sm.cy[cj + mc_dy(ca)], sm.cy[cj + mc_dy(cb)]);
+          const double pz = lerp_axis<M>(cz32, mu, czs[c + mc_dz(ca)], czs[c + mc_dz(cb)]);
+          if (vb + q < totv) {
+            Out* out = verts + (vb + q) * 3;
+            out[0] = (Out)px;
+            out[1] = (Out)py;
+            out[2] = (Out)pz;
+          }
+        }
+        const uint64_t tw = sm.lut_t[cs];
+        const int nf = (int)(tw & 0x7);
+        for (int q = 0; q < nf; ++q) {
+          const uint64_t w = tw >> (3 + 12 * q);
+          if (fb + q < totf) {
+            int64_t* out = faces + (fb + q) * 3;
+            out[0] = vb + 1 + (int64_t)(w & 0xF);
+            out[1] = vb + 1 + (int64_t)((w >> 4) & 0xF);
+            out[2] = vb + 1 + (int64_t)((w >> 8) & 0xF);
+          }
+        }
+      } else if constexpr (METHOD == MT) {
+        const int cls = (gi == 0) | ((gj == 0) << 1) | ((gk == 0) << 2);
+        const int nv = MT_NOWN[cls * 256 + cs];
+        for (int q = 0; q < nv; ++q) {
+          const uint32_t e = MT_OWN_EDGES[(cls * 256 + cs) * 20 + q];
+          const uint32_t ew = MT_EDGE[e];
+          const int ca = ew & 0xF, cb = ew >> 4;
+          const T sv = pc[mt_dz(ca) * PS + mt_dy(ca) * SX + mt_dx(ca)];
+          const T tgv = pc[mt_dz(cb) * PS + mt_dy(cb) * SX + mt_dx(cb)];
+          const M rr = (iso - (M)sv) / (M)(tgv - sv);
+          const A aw = jl_min<A>(jl_max<A>((A)rr, (A)a.eps), (A)a.ome);
+          const A bw = (A)1 - aw;
+          const A wx = (A)mt_dx(ca) * bw + (A)mt_dx(cb) * aw;
+          const A wy = (A)mt_dy(ca) * bw + (A)mt_dy(cb) * aw;
+          const A wz = (A)mt_dz(ca) * bw + (A)mt_dz(cb) * aw;
+          const double px = mt_axis<A>(cx32, wx, sm.cx[ci], sm.cx[ci + 1]);
+          const double py = mt_axis<A>(cy32, wy, sm.cy[cj], sm.cy[cj + 1]);
+          const double pz = mt_axis<A>(cz32, wz, czs[c + H], czs[c + H + 1]);
+          if (vb + q < totv) {
+            Out* out = verts + (vb + q) * 3;
+            out[0] = (Out)px;
+            out[1] = (Out)py;
+            out[2] = (Out)pz;
+          }
+        }
+        const int nf = MT_NF[cs];
+        for (int q = 0; q < nf; ++q) {
+          int64_t id[3];
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            const uint32_t e = MT_FACE[cs * 36 + 3 * q + m];
+            const int qc = MT_EDGE[e] >> 4;  // end corner
+            const int dx = (mt_dx(qc) == 0 && gi > 0), dy = (mt_dy(qc) == 0 && gj > 0),
+                      dz = (mt_dz(qc) == 0 && gk > 0);
+            const uint32_t e2 = MT_SHIFT[e * 8 + (dx | (dy << 1) | (dz << 2))];
+            const int r2 = (cj - dy) * RX + (ci - dx);
+            const int z2 = c - dz;
+            const int cls2 = (gi - dx == 0) | ((gj - dy == 0) << 1) | ((gk - dz == 0) << 2);
+            uint32_t oc;
+            int64_t off;
+            if (z2 >= 0) {
+              oc = sm.ccase[r2 * kWave + z2];
+              off = rbv[r2] + sm.cpv[r2 * kWave + z2];
+            } else {
+              oc = sm.pcase[r2];
+              off = rbv[r2] - (int64_t)MT_NOWN[cls2 * 256 + oc];
+            }
+            id[m] = off + (int64_t)MT_RANK[(cls2 * 256 + oc) * 19 + e2] + 1;
+          }
+          if (fb + q < totf) {
+            int64_t* out = faces + (fb + q) * 3;
+            out[0] = id[0];
+            out[1] = id[1];
+            out[2] = id[2];
+          }
+        }
+      } else {
+        // SN vertex: mean of the crossing points over MC edges 1..12,
+        // summed in promote(M, CX) per axis
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        double dx = 0.0, dy = 0.0, dz = 0.0;
+        const bool sx = sizeof(M) == 4 && cx32, sy = sizeof(M) == 4 && cy32,
+                   sz = sizeof(M) == 4 && cz32;
+        int n = 0;
+        for (int e = 0; e < 12; ++e) {
+          const uint32_t ew = sm.lut_e[e];
+          const int ca = ew & 0xF, cb = ew >> 4;
+          if (((cs >> ca) & 1) == ((cs >> cb) & 1)) continue;
+          ++n;
+          const T va = pc[mc_dz(ca) * PS + mc_dy(ca) * SX + mc_dx(ca)];
+          const T vbv = pc[mc_dz(cb) * PS + mc_dy(cb) * SX + mc_dx(cb)];
+          const M mu = (iso - (M)va) / (M)(vbv - va);
+          const double qx = lerp_axis<M>(cx32, mu, sm.cx[ci + mc_dx(ca)], sm.cx[ci + mc_dx(cb)]);
+          const double qy = lerp_axis<M>(cy32, mu, sm.cy[cj + mc_dy(ca)], sm.cy[cj + mc_dy(cb)]);
+          const double qz =
+              lerp_axis<M>(cz32, mu, czs[c + H + mc_dz(ca)], czs[c + H + mc_dz(cb)]);
+          if (sx) fx = fx + (float)qx; else dx = dx + qx;
+          if (sy) fy = fy + (float)qy; else dy = dy + qy;
+          if (sz) fz = fz + (float)qz; else dz = dz + qz;
+        }
+        if (vb < totv) {
+          Out* out = verts + vb * 3;
+          out[0] = (Out)(sx ? (double)(fx / (float)n) : dx / (double)n);
+          out[1] = (Out)(sy ? (double)(fy / (float)n) : dy / (double)n);
+          out[2] = (Out)(sz ? (double)(fz / (float)n) : dz / (double)n);
+        }
+        // SN faces: one quad per crossing axis edge at the first corner
+        const uint32_t c0 = cs & 1;
+        const int64_t me = vb + 1;
+        int q = 0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+          const uint32_t other = ax == 0 ? (cs >> 1) & 1 : ax == 1 ? (cs >> 3) & 1 : (cs >> 4) & 1;
+          const int u = (ax + 1) % 3, w = (ax + 2) % 3;
+          const int gidx[3] = {gi, gj, gk};
+          if (c0 == other || gidx[u] < 1 || gidx[w] < 1) continue;
+          auto cell_id = [&](int du, int dw) -> int64_t {
+            int d[3] = {0, 0, 0};
+            d[u] = du;
+            d[w] = dw;
+            const int r2 = (cj - d[1]) * RX + (ci - d[0]);
+            const int z2 = c - d[2];
+            if (z2 >= 0) return rbv[r2] + sm.cpv[r2 * kWave + z2] + 1;
+            return rbv[r2];  // the active cell just below the chunk
+          };
+          const int64_t q1 = cell_id(1, 0), q2 = cell_id(1, 1), q3 = cell_id(0, 1);
+          const int64_t a1 = c0 ? q1 : q3, a3 = c0 ? q3 : q1;
+          if (fb + q < totf) {
+            int64_t* out = faces + (fb + q) * 3;
+            out[0] = me;
+            out[1] = a1;
+            out[2] = q2;
+          }
+          if (fb + q + 1 < totf) {
+            int64_t* out = faces + (fb + q + 1) * 3;
+            out[0] = me;
+            out[1] = q2;
+            out[2] = a3;
+          }
+          q += 2;
+        }
+      }
+    };
+
+    // Phase 2: classify (MC) or re-read (MT, SN) the rows this block emits,
+    // scan each row, and collect its active cells; emit a full list with
+    // one cell per lane.  Two rows are classified together so their LDS
+    // and DPP latencies overlap.
+    struct RowScan {
+      uint32_t cs, packed, inc, tot;
+      bool emit;
+    };
+    auto scan_row = [&](int r) {
+      RowScan o{0u, 0u, 0u, 0u, false};
+      const int ci = r % RX, cj = r / RX;
+      const int gi = x0 + ci, gj = y0 + cj;
+      if (ci < H || cj < H || gi >= g.nx - 1 || gj >= g.ny - 1) return o;
+      o.emit = true;
+      uint32_t cs, nv, nf;
+      if constexpr (METHOD == MC) {
+        const T* p = smp + cj * SX + ci;
+        const uint32_t lo = face_code<MC, T, SX>(p + lane * PS, thr);
+        const uint32_t top = face_code<MC, T, SX>(p + kWave * PS, thr);
+        const uint32_t hi = next_lane(lo, top);
+        cs = zvalid ? (lo | (hi << 4)) : 0u;
+        nv = (uint32_t)(sm.lut_v[cs] & 0xF);
+        nf = (uint32_t)(sm.lut_t[cs] & 0x7);
+      } else if constexpr (METHOD == MT) {
+        cs = sm.ccase[r * kWave + lane];
+        const int cls = (gi == 0) | ((gj == 0) << 1) | ((kc == 0) << 2);
+        nv = MT_NOWN[cls * 256 + cs];
+        nf = MT_NF[cs];
+      } else {
+        cs = sm.ccase[r * kWave + lane];
+        nv = (cs != 0 && cs != 255) ? 1u : 0u;
+        nf = 0;
+        if (nv) {
+          const uint32_t c0 = cs & 1;
+          nf += (c0 != ((cs >> 1) & 1) && gj >= 1 && kc >= 1) ? 2 : 0;
+          nf += (c0 != ((cs >> 3) & 1) && kc >= 1 && gi >= 1) ? 2 : 0;
+          nf += (c0 != ((cs >> 4) & 1) && gi >= 1 && gj >= 1) ? 2 : 0;
+        }
+      }
+      o.cs = cs;
+      o.packed = nv | (nf << 16);
+      o.inc = wave_scan_dpp(o.packed);
+      o.tot = (uint32_t)__builtin_amdgcn_readlane((int)o.inc, kWave - 1);
+      return o;
+    };
+    int n = 0;  // wave-uniform list length
+    auto take_row = [&](int r, const RowScan& o) {
+      if (lane == 0) {
+        if (!H) rbv_next[r] = rbv[r] + (o.tot & 0xFFFF);  // MT/SN: set in phase 1
+        rbf_next[r] = rbf[r] + (o.tot >> 16);
+      }
+      if (o.tot == 0) return;
+      const uint32_t ex = o.inc - o.packed;
+      const bool active = o.packed != 0;
+      const uint64_t bal = __ballot(active);
+      const int cnt = __popcll(bal);
+      if (n + cnt > CAP) {  // list full: emit it first
+        wave_sync();
+        for (int k = lane; k < n; k += kWave) emit_cell(list[k]);
+        wave_sync();
+        n = 0;
+      }
+      if (active)
+        list[n + lanes_below(bal)] = (uint64_t)r | ((uint64_t)lane << 8) |
+                                     ((uint64_t)o.cs << 14) | ((uint64_t)(ex & 0xFFFF) << 22) |
+                                     ((uint64_t)(ex >> 16) << 32);
+      n += cnt;
+    };
+    for (int r = wv; r < NR; r += 2 * kWaves) {
+      const RowScan o0 = scan_row(r);
+      const bool two = r + kWaves < NR;
+      const RowScan o1 = two ? scan_row(r + kWaves) : RowScan{0u, 0u, 0u, 0u, false};
+      take_row(r, o0);
+      if (two) take_row(r + kWaves, o1);
+    }
+    // The next brick must have landed before the barrier that ends this
+    // chunk; wait for it here, before this chunk's last stores are issued,
+    // so the wait does not also cover them.
+    __builtin_amdgcn_s_waitcnt(0);
+    wave_sync();
+    for (int k = lane; k < n; k += kWave) emit_cell(list[k]);
+    lds_barrier();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic fields (no reference counterpart): column-major, x fastest.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
+
+__device__ __forceinline__ double lin(int64_t i, int64_t n, double a, double b) {
+  const int64_t d = n - 1 > 1 ? n - 1 : 1;
+  const double t = (double)i / (double)d;
+  return (1.0 - t) * a + t * b;
+}
+
+// Trilinear value noise on an L^3 lattice of seeded Uniform(-1,1) values
+// spanning the unit cube.
+__device__ double value_noise(int64_t i, int64_t j, int64_t k, int64_t nx, int64_t ny,
+                              int64_t nz, int L, uint64_t seed) {
+  const int64_t idx[3] = {i, j, k}, n[3] = {nx, ny, nz};
+  int lo[3];
+  double fr[3];
+  for (int q = 0; q < 3; ++q) {
+    const int64_t d = n[q] - 1 > 1 ? n[q] - 1 : 1;
+    const double u = (double)(idx[q] * (L - 1)) / (double)d;
+    int a = (int)floor(u);
+    if (a > L - 2) a = L - 2;
+    if (a < 0) a = 0;
+    lo[q] = a;
+    fr[q] = u - (double)a;
+  }
+  double acc = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+    const uint64_t li = (uint64_t)(lo[0] + dx) + (uint64_t)L * ((uint64_t)(lo[1] + dy) +
+                                                               (uint64_t)L * (uint64_t)(lo[2] + dz));
+    const double val = 2.0 * u01(splitmix64(seed ^ (li * 0xD1B54A32D192ED03ull))) - 1.0;
+    const double w = (dx ? fr[0] : 1.0 - fr[0]) * (dy ? fr[1] : 1.0 - fr[1]) *
+                     (dz ? fr[2] : 1.0 - fr[2]);
+    acc = acc + w * val;
+  }
+  return acc;
+}
+
+template <typename T>
+__global__ void k_generate(T* __restrict__ out, int kind, int64_t nx, int64_t ny, int64_t nz,
+                           uint64_t seed, double p0, double p1) {
+  const int64_t n = nx * ny * nz;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = q % nx, j = (q / nx) % ny, k = q / (nx * ny);
+    double v;
+    if (kind == 0) {  // sphere
+      const double x = lin(i, nx, -1, 1), y = lin(j, ny, -1, 1), z = lin(k, nz, -1, 1);
+      v = sqrt(x * x + y * y + z * z) - p0;
+    } else if (kind == 1) {  // torus
+      const double x = lin(i, nx, -1, 1), y = lin(j, ny, -1, 1), z = lin(k, nz, -1, 1);
+      const double qq = sqrt(x * x + y * y) - p0;
+      v = sqrt(qq * qq + z * z) - p1;
+    } else if (kind == 2) {  // gyroid on [-p0, p0]^3
+      const double x = lin(i, nx, -p0, p0), y = lin(j, ny, -p0, p0), z = lin(k, nz, -p0, p0);
+      v = cos(x) * sin(y) + cos(y) * sin(z) + cos(z) * sin(x);
+    } else if (kind == 3) {  // value noise
+      v = value_noise(i, j, k, nx, ny, nz, 32, seed) +
+          0.25 * value_noise(i, j, k, nx, ny, nz, 128, seed ^ 0xA5A5A5A5A5A5A5A5ull);
+    } else {  // 16 Gaussian blobs
+      const double x = lin(i, nx, -1, 1), y = lin(j, ny, -1, 1), z = lin(k, nz, -1, 1);
+      double acc = 0.0;
+      for (int b = 0; b < 16; ++b) {
+        const uint64_t h = seed * 16 + (uint64_t)b;
+        const double cx = 1.2 * u01(splitmix64(4 * h + 0)) - 0.6;
+        const double cy = 1.2 * u01(splitmix64(4 * h + 1)) - 0.6;
+        const double cz = 1.2 * u01(splitmix64(4 * h + 2)) - 0.6;
+        const double sg = 0.05 + 0.15 * u01(splitmix64(4 * h + 3));
+        const double d2 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+        acc = acc + exp(-d2 / (2.0 * sg * sg));
+      }
+      v = 0.5 - acc;
+    }
+    out[q] = (T)v;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+
+hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream_t st) {
+  const int n = g.nx + g.ny + g.nz;
+  if (n == 0) return hipSuccess;
+  const int mask = (p.cx[0] == K_F32) | ((p.cx[1] == K_F32) << 1) | ((p.cx[2] == K_F32) << 2);
+  hipLaunchKernelGGL(k_coords, dim3((n + 255) / 256), dim3(256), 0, st, coords, g.nx, g.ny,
+                     g.nz, p.a[0], p.b[0], p.a[1], p.b[1], p.a[2], p.b[2], mask);
+  return hipGetLastError();
+}
+
+template <typename T, int METHOD>
+static hipError_t count_t(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
+                          hipStream_t st) {
+  constexpr int TY = 4, ZU = 8;
+  const int nbx = (g.nx - 1 + kWave - 2) / (kWave - 1);
+  const int nby = (g.ny - 1 + kWaves * TY - 1) / (kWaves * TY);
+  const int64_t nb = (int64_t)nbx * nby * g.nzseg;
+  hipLaunchKernelGGL((k_count<T, METHOD, TY, ZU>), dim3((unsigned)nb), dim3(kBlock), 0, st,
+                     (const T*)field, g, (T)p.thr, cnt, nbx, nby, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_count(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
+                        hipStream_t st) {
+  if (p.T == K_F32) {
+    if (p.method == MC) return count_t<float, MC>(p, g, field, cnt, st);
+    if (p.method == MT) return count_t<float, MT>(p, g, field, cnt, st);
+    return count_t<float, SN>(p, g, field, cnt, st);
+  }
+  if (p.method == MC) return count_t<double, MC>(p, g, field, cnt, st);
+  if (p.method == MT) return count_t<double, MT>(p, g, field, cnt, st);
+  return count_t<double, SN>(p, g, field, cnt, st);
+}
+
+hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st) {
+  const int64_t nt = scan_tiles(g.units);
+  // status[nt] and the ticket counter are contiguous: status | counter
+  const int64_t nz = nt + 1;
+  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st,
+                     w.status, nz);
+  hipLaunchKernelGGL(k_scan, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt, g.units, w.voff,
+                     w.foff, w.status, w.incl, w.counter, w.totals, nt);
+  return hipGetLastError();
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out, int TX, int TY>
+static hipError_t emit_t(const EmitArgs& a, void* verts, int64_t* faces, hipStream_t st) {
+  EmitArgs b = a;
+  b.nti = (a.g.nx - 1 + TX - 1) / TX;
+  b.ntj = (a.g.ny - 1 + TY - 1) / TY;
+  b.ntiles = (int64_t)b.nti * b.ntj * a.g.nzseg;
+  hipLaunchKernelGGL((k_emit<METHOD, T, M, A, Out, TX, TY>), dim3((unsigned)b.ntiles),
+                     dim3(kBlock), 0, st, b, (Out*)verts, faces);
+  return hipGetLastError();
+}
+
+// Brick shape (cells along x, y) of the emit kernel.  MSH_EMIT_SHAPE picks
+// one of the Marching Cubes shapes (0: 31x2, 1: 31x1, 2: 15x2, 3: 15x1) for
+// tuning; the default is per element type.
+static int emit_shape_choice(int dflt) {
+  const char* e = getenv("MSH_EMIT_SHAPE");
+  if (!e || !*e) return dflt;
+  const int v = atoi(e);
+  return (v >= 0 && v <= 3) ? v : dflt;
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out>
+static hipError_t emit_shape(const EmitArgs& a, void* verts, int64_t* faces, hipStream_t st) {
+  if constexpr (METHOD == MC) {
+    switch (emit_shape_choice(sizeof(T) == 4 ? 1 : 3)) {
+      case 0: return emit_t<MC, T, M, A, Out, 31, 2>(a, verts, faces, st);
+      case 1: return emit_t<MC, T, M, A, Out, 31, 1>(a, verts, faces, st);
+      case 2: return emit_t<MC, T, M, A, Out, 15, 2>(a, verts, faces, st);
+      default: return emit_t<MC, T, M, A, Out, 15, 1>(a, verts, faces, st);
+    }
+  } else if constexpr (sizeof(T) == 4) {
+    return emit_t<METHOD, T, M, A, Out, 31, 1>(a, verts, faces, st);
+  } else {
+    return emit_t<METHOD, T, M, A, Out, 15, 1>(a, verts, faces, st);
+  }
+}
+
+template <int METHOD>
+static hipError_t emit_types(const Plan& p, const EmitArgs& a, void* verts, int64_t* faces,
+                             hipStream_t st) {
+  if (p.T == K_F32 && p.M == K_F32 && p.A == K_F32 && p.Out == K_F32)
+    return emit_shape<METHOD, float, float, float, float>(a, verts, faces, st);
+  if (p.T == K_F32 && p.M == K_F32 && p.A == K_F32)
+    return emit_shape<METHOD, float, float, float, double>(a, verts, faces, st);
+  if (p.T == K_F32 && p.M == K_F32)
+    return emit_shape<METHOD, float, float, double, double>(a, verts, faces, st);
+  if (p.T == K_F32) return emit_shape<METHOD, float, double, double, double>(a, verts, faces, st);
+  return emit_shape<METHOD, double, double, double, double>(a, verts, faces, st);
+}
+
+hipError_t launch_emit(const Plan& p, const Geom& g, const void* field, const Workspace& w,
+                       void* verts, int64_t* faces, hipStream_t st) {
+  EmitArgs a{};
+  a.f = field;
+  a.g = g;
+  a.thr = p.thr;
+  a.iso = p.isoM;
+  a.eps = p.epsA;
+  a.ome = p.omeA;
+  a.cxmask = (p.cx[0] == K_F32) | ((p.cx[1] == K_F32) << 1) | ((p.cx[2] == K_F32) << 2);
+  a.coords = w.coords;
+  a.voff = w.voff;
+  a.foff = w.foff;
+  a.totals = w.totals;
+  if (p.method == MC) {
+    Plan q = p;
+    q.A = q.M;  // no eps in Marching Cubes
+    return emit_types<MC>(q, a, verts, faces, st);
+  }
+  if (p.method == SN) {
+    Plan q = p;
+    q.A = q.M;
+    return emit_types<SN>(q, a, verts, faces, st);
+  }
+  return emit_types<MT>(p, a, verts, faces, st);
+}
+
+hipError_t launch_generate(int kind, int dtype, int64_t nx, int64_t ny, int64_t nz,
+                           uint64_t seed, double p0, double p1, void* out, hipStream_t st) {
+  const int64_t n = nx * ny * nz;
+  if (n == 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (dtype == K_F32)
+    hipLaunchKernelGGL(k_generate<float>, dim3((unsigned)blocks), dim3(256), 0, st, (float*)out,
+                       kind, nx, ny, nz, seed, p0, p1);
+  else
+    hipLaunchKernelGGL(k_generate<double>, dim3((unsigned)blocks), dim3(256), 0, st,
+                       (double*)out, kind, nx, ny, nz, seed, p0, p1);
+  return hipGetLastError();
+}
+
+}  // namespace msh
