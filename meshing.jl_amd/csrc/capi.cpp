@@ -83,6 +83,9 @@ int make_plan(Ctx* c, const msh_field* f, const msh_params* p, Plan* out) {
     return fail(c, MSH_ERR_ARGUMENT, "negative array size");
   if (f->nx > INT32_MAX / 2 || f->ny > INT32_MAX / 2 || f->nz > INT32_MAX / 2)
     return fail(c, MSH_ERR_ARGUMENT, "array dimension too large");
+  // the kernels address one z-plane with 32-bit byte offsets
+  if ((double)f->nx * (double)f->ny * (f->dtype == K_F64 ? 8.0 : 4.0) >= 2147483648.0)
+    return fail(c, MSH_ERR_UNSUPPORTED, "one z-plane of the field must be smaller than 2 GiB");
   if (p->method != MSH_MARCHING_CUBES && p->method != MSH_MARCHING_TETRAHEDRA &&
       p->method != MSH_NAIVE_SURFACE_NETS)
     return fail(c, MSH_ERR_ARGUMENT, "unknown meshing algorithm");
@@ -172,10 +175,14 @@ Geom make_geom(const msh_field* f) {
     return g;
   }
   // Split z rows into segments when the xy tiling alone gives too few
-  // blocks to fill 256 CUs; segments of at most 4096 cells keep the packed
+  // blocks to fill 256 CUs (the count kernel has the fewest: 252 x-cells by
+  // 6-8 rows per block); segments of at most 4096 cells keep the packed
   // 16-bit per-unit counts from overflowing.
-  const int64_t tiles_xy = ((cx + 30) / 31) * ((cy + 3) / 4);
-  int64_t nseg = (2048 + tiles_xy - 1) / tiles_xy;
+  const int64_t ty = f->dtype == K_F64 ? 6 : 8;
+  const int64_t blocks_xy = ((cx + 251) / 252) * ((cy + ty - 1) / ty);
+  // many long z runs stream better than many short ones: split only when
+  // there are fewer than ~2 count blocks per CU
+  int64_t nseg = (512 + blocks_xy - 1) / blocks_xy;
   const int64_t max_seg = (cz + 63) / 64;
   if (nseg > max_seg) nseg = max_seg;
   if (nseg < 1) nseg = 1;

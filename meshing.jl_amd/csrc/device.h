@@ -1,0 +1,180 @@
+// device.h -- device helpers shared by the kernels of libmeshing_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+
+#include "internal.h"
+
+namespace msh {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / kWave;
+
+// Corner offsets.  Marching Cubes order (src/marching_cubes.jl:42-49):
+// (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1).
+__device__ __forceinline__ int mc_dx(int c) { return (0x66 >> c) & 1; }
+__device__ __forceinline__ int mc_dy(int c) { return (0xCC >> c) & 1; }
+__device__ __forceinline__ int mc_dz(int c) { return (0xF0 >> c) & 1; }
+// Marching Tetrahedra order (src/marching_tetrahedra.jl:146-153):
+// (0,0,0) (0,1,0) (1,1,0) (1,0,0) (0,0,1) (0,1,1) (1,1,1) (1,0,1).
+__device__ __forceinline__ int mt_dx(int c) { return (0xCC >> c) & 1; }
+__device__ __forceinline__ int mt_dy(int c) { return (0x66 >> c) & 1; }
+__device__ __forceinline__ int mt_dz(int c) { return (0xF0 >> c) & 1; }
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+// Wave index inside the block, in a scalar register (it is the same for all
+// lanes; saying so keeps address math that depends on it scalar).
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+}
+
+// Make this wave's LDS writes visible to its own later LDS reads.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint64_t t = __shfl_up(v, d, kWave);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Workgroup barrier ordering LDS only: it does not wait for this wave's
+// outstanding global stores, which __syncthreads() would.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Lane i gets lane i+1's value (DPP wave_shl:1); lane 63 gets `last`.
+__device__ __forceinline__ uint32_t next_lane(uint32_t v, uint32_t last) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)v, 0x130, 0xF, 0xF, false);
+}
+
+// Blocks of one launch are dealt round-robin to the 8 XCDs (each with its
+// own L2).  Renumber so that each XCD gets a contiguous range of tiles:
+// neighbouring tiles (which share halo samples) then meet in the same L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+  const int64_t xcd = b & 7, local = b >> 3;
+  const int64_t per = nb >> 3, rem = nb & 7;
+  return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
+
+struct EmitArgs {
+  const void* f;
+  Geom g;
+  double thr, iso;         // threshold in T, iso in M
+  double eps, ome;         // MT: eps and one(T)-eps, in A
+  int cxmask;              // bit k: axis k coordinates are Float32
+  const double* coords;    // xp | yp | zp
+  const int64_t* voff;
+  const int64_t* foff;
+  const int64_t* totals;   // (nverts, nfaces): writes beyond are dropped
+  int nti, ntj;            // tiles along x and y
+  int64_t ntiles;
+};
+
+// Inclusive scan over the 64 lanes with DPP: Kogge-Stone within each row of
+// 16 lanes, then row broadcasts of lanes 15 and 31.
+__device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return v;
+}
+
+// One axis of p1 .+ mu .* (p2 .- p1) (src/marching_cubes.jl:100-104) with
+// p1, p2 of the axis element type; the result type is promote(M, CX).
+template <typename M>
+__device__ __forceinline__ double lerp_axis(bool cx32, M mu, double p1, double p2) {
+  if (cx32) {
+    const float a = (float)p1, b = (float)p2;
+    return (double)(a + mu * (b - a));
+  }
+  return (double)(p1 + mu * (p2 - p1));
+}
+
+// One axis of vertPos: base .+ (c1*b .+ c2*a) .* (next - base)
+// (src/marching_tetrahedra.jl:49-54), with w = c1*b + c2*a of type A.
+template <typename A>
+__device__ __forceinline__ double mt_axis(bool cx32, A w, double lo, double hi) {
+  if (cx32) {
+    const float l = (float)lo, h = (float)hi;
+    const float d = h - l;
+    return (double)(l + w * d);
+  }
+  const double d = hi - lo;
+  return (double)(lo + w * d);
+}
+
+// Julia's min/max of floats: NaN-propagating, -0.0 < +0.0.
+template <typename C>
+__device__ __forceinline__ C jl_min(C x, C y) {
+  if (isnan(x)) return x;
+  if (isnan(y)) return y;
+  return signbit(x - y) ? x : y;
+}
+template <typename C>
+__device__ __forceinline__ C jl_max(C x, C y) {
+  if (isnan(x)) return x;
+  if (isnan(y)) return y;
+  return signbit(x - y) ? y : x;
+}
+
+
+// 4-bit code of the sample plane under a cell's lower face, in the corner
+// order of METHOD.  p points at sample (ci, cj) of that plane.
+template <int METHOD, typename T, int SX>
+__device__ __forceinline__ uint32_t face_code(const T* p, T thr) {
+  const uint32_t b00 = p[0] < thr, b10 = p[1] < thr, b11 = p[SX + 1] < thr, b01 = p[SX] < thr;
+  if (METHOD == MT) return b00 | (b01 << 1) | (b11 << 2) | (b10 << 3);
+  return b00 | (b10 << 1) | (b11 << 2) | (b01 << 3);
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+
+// Inclusive scan over independent segments of SEG lanes (16, 32 or 64) with
+// DPP: Kogge-Stone within each row of 16 lanes, then row broadcasts.
+template <int SEG>
+__device__ __forceinline__ uint32_t seg_scan_dpp(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  if (SEG >= 32)
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  if (SEG >= 64)
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return v;
+}
+
+// Raw buffer resource over `bytes` bytes at p (32-bit data format, so
+// loads past `bytes` return 0 instead of touching memory).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+}
+
+hipError_t launch_count4(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
+                         hipStream_t st);
+hipError_t launch_emit_mc4(const Plan& p, const EmitArgs& a, void* verts, int64_t* faces,
+                           hipStream_t st);
+
+}  // namespace msh

@@ -21,68 +21,13 @@
 #include <stdlib.h>
 
 #include "internal.h"
+#include "device.h"
 
 #define MSH_LUT static __constant__
 #include "luts.h"
 
 namespace msh {
 namespace {
-
-constexpr int kWave = 64;
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / kWave;
-
-// Corner offsets.  Marching Cubes order (src/marching_cubes.jl:42-49):
-// (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1).
-__device__ __forceinline__ int mc_dx(int c) { return (0x66 >> c) & 1; }
-__device__ __forceinline__ int mc_dy(int c) { return (0xCC >> c) & 1; }
-__device__ __forceinline__ int mc_dz(int c) { return (0xF0 >> c) & 1; }
-// Marching Tetrahedra order (src/marching_tetrahedra.jl:146-153):
-// (0,0,0) (0,1,0) (1,1,0) (1,0,0) (0,0,1) (0,1,1) (1,1,1) (1,0,1).
-__device__ __forceinline__ int mt_dx(int c) { return (0xCC >> c) & 1; }
-__device__ __forceinline__ int mt_dy(int c) { return (0x66 >> c) & 1; }
-__device__ __forceinline__ int mt_dz(int c) { return (0xF0 >> c) & 1; }
-
-__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
-
-// Make this wave's LDS writes visible to its own later LDS reads.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const uint64_t t = __shfl_up(v, d, kWave);
-    if (lane >= d) v += t;
-  }
-  return v;
-}
-
-// Workgroup barrier ordering LDS only: it does not wait for this wave's
-// outstanding global stores, which __syncthreads() would.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// Lane i gets lane i+1's value (DPP wave_shl:1); lane 63 gets `last`.
-__device__ __forceinline__ uint32_t next_lane(uint32_t v, uint32_t last) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)v, 0x130, 0xF, 0xF, false);
-}
-
-// Blocks of one launch are dealt round-robin to the 8 XCDs (each with its
-// own L2).  Renumber so that each XCD gets a contiguous range of tiles:
-// neighbouring tiles (which share halo samples) then meet in the same L2.
-__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
-  const int64_t xcd = b & 7, local = b >> 3;
-  const int64_t per = nb >> 3, rem = nb & 7;
-  return xcd * per + (xcd < rem ? xcd : rem) + local;
-}
 
 // ---------------------------------------------------------------------------
 // LinRange coordinates: xp[i] = CX((1-t)*a + t*b), t = (i-1)/max(1, n-1)
@@ -107,121 +52,6 @@ __global__ void k_coords(double* __restrict__ out, int nx, int ny, int nz, doubl
   double v = (1.0 - t) * a + t * b;
   if ((cxmask >> axis) & 1) v = (double)(float)v;
   out[i] = v;
-}
-
-// ---------------------------------------------------------------------------
-// Count.  Block = 4 waves; wave w covers sample columns x in [xb, xb+64)
-// (cells xb..xb+62) and TY rows of cells starting at yb; it marches the z
-// segment of blockIdx.z.  Per plane it loads TY+1 samples per lane, turns
-// them into a bit mask (bit r = sample row r is below iso), and gets the
-// x+1 column's mask from the next lane.
-template <int METHOD>
-__device__ __forceinline__ uint32_t plane_code(uint32_t m, uint32_t mn, int r) {
-  const uint32_t b00 = (m >> r) & 1, b01 = (m >> (r + 1)) & 1;   // (x, y), (x, y+1)
-  const uint32_t b10 = (mn >> r) & 1, b11 = (mn >> (r + 1)) & 1; // (x+1, y), (x+1, y+1)
-  if (METHOD == MT) return b00 | (b01 << 1) | (b11 << 2) | (b10 << 3);
-  return b00 | (b10 << 1) | (b11 << 2) | (b01 << 3);
-}
-
-template <typename T, int METHOD, int TY, int ZU>
-__global__ __launch_bounds__(kBlock) void k_count(const T* __restrict__ f, Geom g, T thr,
-                                                  uint32_t* __restrict__ cnt, int nbx, int nby,
-                                                  int64_t nblocks) {
-  __shared__ uint16_t lut[METHOD == MT ? 2048 : 256];
-  if (METHOD == MC) {
-    for (int c = threadIdx.x; c < 256; c += kBlock)
-      lut[c] = (uint16_t)((MC_CASE_VERTS[c] & 0xF) | ((MC_CASE_TRIS[c] & 0x7) << 8));
-  } else if (METHOD == MT) {
-    for (int c = threadIdx.x; c < 2048; c += kBlock)
-      lut[c] = (uint16_t)(MT_NOWN[c] | (MT_NF[c & 255] << 8));
-  }
-  __syncthreads();
-
-  const int lane = lane_id();
-  const int wv = threadIdx.x / kWave;
-  // y-neighbouring tiles (which share a halo row of samples) run on the
-  // same XCD, so the shared row is read from HBM once.
-  const int64_t t = xcd_tile(blockIdx.x, nblocks);
-  const int by = (int)(t % nby);
-  const int64_t rest = t / nby;
-  const int bx = (int)(rest % nbx);
-  const int s = (int)(rest / nbx);
-  const int xb = bx * (kWave - 1);
-  const int yb = (by * kWaves + wv) * TY;
-  if (yb >= g.ny - 1) return;
-  const int zs = s * g.zseg;
-  const int ze = min(zs + g.zseg, g.nz - 1);
-  const int xs = min(xb + lane, g.nx - 1);
-  const bool xvalid = lane < kWave - 1 && xb + lane < g.nx - 1;
-  const int64_t pstride = (int64_t)g.nx * g.ny;
-
-  int64_t roff[TY + 1];
-#pragma unroll
-  for (int r = 0; r <= TY; ++r) roff[r] = (int64_t)min(yb + r, g.ny - 1) * g.nx + xs;
-
-  auto mask_of = [&](int kk) {
-    const T* p = f + (int64_t)kk * pstride;
-    uint32_t m = 0;
-#pragma unroll
-    for (int r = 0; r <= TY; ++r) m |= (uint32_t)(p[roff[r]] < thr) << r;
-    return m;
-  };
-
-  uint32_t acc[TY];
-#pragma unroll
-  for (int r = 0; r < TY; ++r) acc[r] = 0;
-
-  uint32_t m0 = mask_of(zs);
-  uint32_t m0n = next_lane(m0, 0u);
-  for (int k = zs; k < ze; k += ZU) {
-    T v[ZU][TY + 1];
-#pragma unroll
-    for (int u = 0; u < ZU; ++u) {
-      const T* p = f + (int64_t)min(k + 1 + u, g.nz - 1) * pstride;
-#pragma unroll
-      for (int r = 0; r <= TY; ++r) v[u][r] = p[roff[r]];
-    }
-#pragma unroll
-    for (int u = 0; u < ZU; ++u) {
-      if (k + u >= ze) break;
-      uint32_t m1 = 0;
-#pragma unroll
-      for (int r = 0; r <= TY; ++r) m1 |= (uint32_t)(v[u][r] < thr) << r;
-      const uint32_t m1n = next_lane(m1, 0u);
-      const int kc = k + u;
-#pragma unroll
-      for (int r = 0; r < TY; ++r) {
-        const uint32_t cs = plane_code<METHOD>(m0, m0n, r) | (plane_code<METHOD>(m1, m1n, r) << 4);
-        if (METHOD == MC) {
-          const uint32_t e = lut[cs];
-          acc[r] += (e & 0xF) | ((e >> 8) << 16);
-        } else if (METHOD == MT) {
-          const int cls = (xb + lane == 0) | ((yb + r == 0) << 1) | ((kc == 0) << 2);
-          const uint32_t e = lut[cls * 256 + cs];
-          acc[r] += (e & 0xFF) | ((e >> 8) << 16);
-        } else {
-          if (cs != 0 && cs != 255) {
-            const uint32_t c0 = cs & 1;
-            const int i = xb + lane, j = yb + r;
-            uint32_t nf = 0;
-            nf += (c0 != ((cs >> 1) & 1) && j >= 1 && kc >= 1) ? 2 : 0;  // x edge
-            nf += (c0 != ((cs >> 3) & 1) && kc >= 1 && i >= 1) ? 2 : 0;  // y edge
-            nf += (c0 != ((cs >> 4) & 1) && i >= 1 && j >= 1) ? 2 : 0;   // z edge
-            acc[r] += 1 | (nf << 16);
-          }
-        }
-      }
-      m0 = m1;
-      m0n = m1n;
-    }
-  }
-  if (!xvalid) return;
-#pragma unroll
-  for (int r = 0; r < TY; ++r) {
-    const int j = yb + r;
-    if (j < g.ny - 1)
-      cnt[((int64_t)(xb + lane) * (g.ny - 1) + j) * g.nzseg + s] = acc[r];
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -260,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ cn
   if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
   __syncthreads();
   const int64_t tile = s_tile;
-  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  const int lane = lane_id(), wv = wave_id();
   const int64_t base = tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
 
   uint32_t c[kScanItems];
@@ -375,70 +205,6 @@ __global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ cn
 // -x, -y, -z.  Those cells' cases and in-row vertex prefixes are in LDS
 // (halo rows, and the plane below the chunk), so a face's vertex id is
 // (row base + prefix of the owning cell + rank of the edge in that cell).
-struct EmitArgs {
-  const void* f;
-  Geom g;
-  double thr, iso;         // threshold in T, iso in M
-  double eps, ome;         // MT: eps and one(T)-eps, in A
-  int cxmask;              // bit k: axis k coordinates are Float32
-  const double* coords;    // xp | yp | zp
-  const int64_t* voff;
-  const int64_t* foff;
-  const int64_t* totals;   // (nverts, nfaces): writes beyond are dropped
-  int nti, ntj;            // tiles along x and y
-  int64_t ntiles;
-};
-
-// Inclusive scan over the 64 lanes with DPP: Kogge-Stone within each row of
-// 16 lanes, then row broadcasts of lanes 15 and 31.
-__device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
-  return v;
-}
-
-// One axis of p1 .+ mu .* (p2 .- p1) (src/marching_cubes.jl:100-104) with
-// p1, p2 of the axis element type; the result type is promote(M, CX).
-template <typename M>
-__device__ __forceinline__ double lerp_axis(bool cx32, M mu, double p1, double p2) {
-  if (cx32) {
-    const float a = (float)p1, b = (float)p2;
-    return (double)(a + mu * (b - a));
-  }
-  return (double)(p1 + mu * (p2 - p1));
-}
-
-// One axis of vertPos: base .+ (c1*b .+ c2*a) .* (next - base)
-// (src/marching_tetrahedra.jl:49-54), with w = c1*b + c2*a of type A.
-template <typename A>
-__device__ __forceinline__ double mt_axis(bool cx32, A w, double lo, double hi) {
-  if (cx32) {
-    const float l = (float)lo, h = (float)hi;
-    const float d = h - l;
-    return (double)(l + w * d);
-  }
-  const double d = hi - lo;
-  return (double)(lo + w * d);
-}
-
-// Julia's min/max of floats: NaN-propagating, -0.0 < +0.0.
-template <typename C>
-__device__ __forceinline__ C jl_min(C x, C y) {
-  if (isnan(x)) return x;
-  if (isnan(y)) return y;
-  return signbit(x - y) ? x : y;
-}
-template <typename C>
-__device__ __forceinline__ C jl_max(C x, C y) {
-  if (isnan(x)) return x;
-  if (isnan(y)) return y;
-  return signbit(x - y) ? y : x;
-}
-
 template <int METHOD, typename T, int TX, int TY>
 struct EmitCfg {
   static constexpr int H = METHOD == MC ? 0 : 1;       // low-side halo
@@ -466,20 +232,6 @@ struct EmitSmem {
   uint64_t list[kWaves][C::CAP];    // active cells: row | z<<8 | case<<14 | pv<<22 | pf<<32
 };
 
-// 4-bit code of the sample plane under a cell's lower face, in the corner
-// order of METHOD.  p points at sample (ci, cj) of that plane.
-template <int METHOD, typename T, int SX>
-__device__ __forceinline__ uint32_t face_code(const T* p, T thr) {
-  const uint32_t b00 = p[0] < thr, b10 = p[1] < thr, b11 = p[SX + 1] < thr, b01 = p[SX] < thr;
-  if (METHOD == MT) return b00 | (b01 << 1) | (b11 << 2) | (b10 << 3);
-  return b00 | (b10 << 1) | (b11 << 2) | (b01 << 3);
-}
-
-__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
 template <int METHOD, typename T, typename M, typename A, typename Out, int TX, int TY>
 __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, Out* __restrict__ verts,
                                                  int64_t* __restrict__ faces) {
@@ -489,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, Out* __restrict__ v
   __shared__ EmitSmem<METHOD, T, TX, TY> sm;
   const T* __restrict__ f = (const T*)a.f;
   const Geom g = a.g;
-  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  const int lane = lane_id(), wv = wave_id();
 
   const int64_t t = xcd_tile(blockIdx.x, a.ntiles);
   const int tj = (int)(t % a.ntj);
@@ -969,28 +721,9 @@ hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream
   return hipGetLastError();
 }
 
-template <typename T, int METHOD>
-static hipError_t count_t(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
-                          hipStream_t st) {
-  constexpr int TY = 4, ZU = 8;
-  const int nbx = (g.nx - 1 + kWave - 2) / (kWave - 1);
-  const int nby = (g.ny - 1 + kWaves * TY - 1) / (kWaves * TY);
-  const int64_t nb = (int64_t)nbx * nby * g.nzseg;
-  hipLaunchKernelGGL((k_count<T, METHOD, TY, ZU>), dim3((unsigned)nb), dim3(kBlock), 0, st,
-                     (const T*)field, g, (T)p.thr, cnt, nbx, nby, nb);
-  return hipGetLastError();
-}
-
 hipError_t launch_count(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
                         hipStream_t st) {
-  if (p.T == K_F32) {
-    if (p.method == MC) return count_t<float, MC>(p, g, field, cnt, st);
-    if (p.method == MT) return count_t<float, MT>(p, g, field, cnt, st);
-    return count_t<float, SN>(p, g, field, cnt, st);
-  }
-  if (p.method == MC) return count_t<double, MC>(p, g, field, cnt, st);
-  if (p.method == MT) return count_t<double, MT>(p, g, field, cnt, st);
-  return count_t<double, SN>(p, g, field, cnt, st);
+  return launch_count4(p, g, field, cnt, st);
 }
 
 hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st) {
@@ -1069,6 +802,11 @@ hipError_t launch_emit(const Plan& p, const Geom& g, const void* field, const Wo
   a.foff = w.foff;
   a.totals = w.totals;
   if (p.method == MC) {
+    const char* legacy = getenv("MSH_EMIT_V3");
+    if (!(legacy && *legacy == '1')) {
+      const hipError_t e = launch_emit_mc4(p, a, verts, faces, st);
+      if (e != hipErrorNotSupported) return e;
+    }
     Plan q = p;
     q.A = q.M;  // no eps in Marching Cubes
     return emit_types<MC>(q, a, verts, faces, st);

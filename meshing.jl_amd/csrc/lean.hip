@@ -1,0 +1,543 @@
+// lean.hip -- the count kernel of all methods and the Marching Cubes emit
+// kernel, written to keep vector-ALU work per cell small (these passes are
+// bound by VALU issue as much as by HBM).
+//
+// Both classify cells from "below iso" bit masks instead of comparing each
+// cell's 8 corners: a sample is compared once, and the 4 bits of a cell
+// face are pulled out of a mask with one shift.  Bits are kept in
+// "natural" order inside each nibble -- (x,y) (x+1,y) (x,y+1) (x+1,y+1) --
+// and the lookup tables are permuted once per block to that order.
+#include "device.h"
+
+#define MSH_LUT static __constant__
+#include "luts.h"
+
+namespace msh {
+namespace {
+
+// Cells per emit tile along x: 60 keeps every tile's first sample 16-byte
+// aligned (when the row pitch is) while 61 samples fit a 64-lane row.
+constexpr int kEmitTX = 60;
+
+// Natural nibble order -> Marching Cubes corner order (src/marching_cubes.jl:42-49):
+// swap bits 2 and 3 of each nibble.
+__device__ __forceinline__ uint32_t nat_to_mc(uint32_t c) {
+  return (c & 0x33u) | ((c & 0x44u) << 1) | ((c & 0x88u) >> 1);
+}
+// Natural nibble order -> Marching Tetrahedra corner order
+// (src/marching_tetrahedra.jl:146-153): (x,y) (x,y+1) (x+1,y+1) (x+1,y).
+__device__ __forceinline__ uint32_t nat_to_mt(uint32_t c) {
+  return (c & 0x11u) | ((c & 0x44u) >> 1) | ((c & 0x88u) >> 1) | ((c & 0x22u) << 2);
+}
+
+// One LDS DMA instruction: each lane copies BYTES bytes from the buffer at
+// byte offset `off` to lds + lane*BYTES.  (The 16-byte form exists only in
+// the gfx950 device compile; the host compile never runs kernels.)
+template <int BYTES>
+__device__ __forceinline__ void dma_to_lds(__amdgpu_buffer_rsrc_t r, uint32_t* lds, uint32_t off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (BYTES == 16)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (void*)lds, 16, off, 0, 0, 0);
+  else
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (void*)lds, 4, off, 0, 0, 0);
+#endif
+}
+
+template <typename T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+// ---------------------------------------------------------------------------
+// Count.  The 4 waves of a block sit side by side along x; wave w covers
+// sample columns [xb, xb+64) (cells xb..xb+62) and TY cell rows from yb, and
+// marches the z segment of its tile.  Per plane a lane loads TY+1 samples
+// (buffer loads: the address is a per-row offset fixed for the whole run
+// plus a per-plane base in scalar registers), packs their below-iso bits at
+// even positions, and ORs in the x+1 column's bits from the next lane at
+// odd positions; cell row r's face nibble is then bits [2r, 2r+4).
+template <typename T, int METHOD, int TY, int ZU>
+__global__ __launch_bounds__(kBlock) void k_count4(const T* __restrict__ f, Geom g, T thr,
+                                                   uint32_t* __restrict__ cnt, int nbx, int nby,
+                                                   int64_t nblocks) {
+  static_assert(2 * (TY + 1) <= 32, "mask must fit 32 bits");
+  constexpr int NL = METHOD == MC ? 256 : 2048;
+  __shared__ uint32_t lut[NL];  // vertices | faces << 16, by class << 8 | natural case
+  for (int c = threadIdx.x; c < NL; c += kBlock) {
+    const uint32_t cs = c & 255, cls = c >> 8;
+    uint32_t v;
+    if (METHOD == MC) {
+      const uint32_t m = nat_to_mc(cs);
+      v = (uint32_t)(MC_CASE_VERTS[m] & 0xF) | ((uint32_t)(MC_CASE_TRIS[m] & 0x7) << 16);
+    } else if (METHOD == MT) {
+      // cls: first x layer | first y layer << 1 | first z layer << 2
+      const uint32_t m = nat_to_mt(cs);
+      v = (uint32_t)MT_NOWN[cls * 256 + m] | ((uint32_t)MT_NF[m] << 16);
+    } else {
+      // cls: x >= 1 | y >= 1 << 1 | z >= 1 << 2 (a quad needs the cells at -1)
+      const uint32_t m = nat_to_mc(cs);
+      const uint32_t nv = (m != 0 && m != 255) ? 1u : 0u;
+      uint32_t nf = 0;
+      if (nv) {
+        const uint32_t c0 = m & 1;
+        nf += (c0 != ((m >> 1) & 1) && (cls & 2) && (cls & 4)) ? 2 : 0;  // x edge
+        nf += (c0 != ((m >> 3) & 1) && (cls & 4) && (cls & 1)) ? 2 : 0;  // y edge
+        nf += (c0 != ((m >> 4) & 1) && (cls & 1) && (cls & 2)) ? 2 : 0;  // z edge
+      }
+      v = nv | (nf << 16);
+    }
+    lut[c] = v;
+  }
+  __syncthreads();
+
+  const int lane = lane_id(), wv = wave_id();
+  const int64_t t = xcd_tile(blockIdx.x, nblocks);
+  const int by = (int)(t % nby);
+  const int64_t rest = t / nby;
+  const int bx = (int)(rest % nbx);
+  const int s = (int)(rest / nbx);
+  const int xb = (bx * kWaves + wv) * (kWave - 1);
+  const int yb = by * TY;
+  if (xb >= g.nx - 1) return;
+  const int zs = s * g.zseg, ze = min(zs + g.zseg, g.nz - 1);
+  const bool xvalid = lane < kWave - 1 && xb + lane < g.nx - 1;
+  const int xs = min(xb + lane, g.nx - 1);
+  const int64_t pstride = (int64_t)g.nx * g.ny;
+  const uint32_t pbytes = (uint32_t)(pstride * (int64_t)sizeof(T));
+
+  uint32_t voff[TY + 1];
+#pragma unroll
+  for (int r = 0; r <= TY; ++r)
+    voff[r] = ((uint32_t)min(yb + r, g.ny - 1) * (uint32_t)g.nx + (uint32_t)xs) * sizeof(T);
+  uint32_t clsx = 0, clsy[TY];
+  if (METHOD == MT) clsx = (xb + lane == 0) ? 1u : 0u;
+  if (METHOD == SN) clsx = (xb + lane >= 1) ? 1u : 0u;
+#pragma unroll
+  for (int r = 0; r < TY; ++r) {
+    clsy[r] = METHOD == MT ? ((yb + r == 0) ? 2u : 0u) : ((yb + r >= 1) ? 2u : 0u);
+    clsy[r] |= clsx;
+  }
+
+  auto plane_bits = [&](const T* v) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) w |= (uint32_t)(v[r] < thr) << (2 * r);
+    return w | (next_lane(w, 0u) << 1);
+  };
+
+  uint32_t acc[TY];
+#pragma unroll
+  for (int r = 0; r < TY; ++r) acc[r] = 0;
+
+  auto load_group = [&](int k, T (&v)[ZU][TY + 1]) {
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      const __amdgpu_buffer_rsrc_t rs =
+          buffer_rsrc(f + (int64_t)min(k + 1 + u, g.nz - 1) * pstride, pbytes);
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) v[u][r] = buffer_load<T>(rs, voff[r]);
+    }
+  };
+  uint32_t w0;
+  auto use_group = [&](int k, const T (&v)[ZU][TY + 1]) {
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      if (k + u >= ze) break;
+      const uint32_t w1 = plane_bits(v[u]);
+      const int kc = k + u;
+      const uint32_t clsz = METHOD == MT ? ((kc == 0) ? 4u : 0u) : ((kc >= 1) ? 4u : 0u);
+#pragma unroll
+      for (int r = 0; r < TY; ++r) {
+        const uint32_t cs = ((w0 >> (2 * r)) & 0xF) | (((w1 >> (2 * r)) & 0xF) << 4);
+        const uint32_t idx = METHOD == MC ? cs : (cs | ((clsy[r] | clsz) << 8));
+        acc[r] += lut[idx];
+      }
+      w0 = w1;
+    }
+  };
+  {
+    T v0[TY + 1];
+    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(f + (int64_t)zs * pstride, pbytes);
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) v0[r] = buffer_load<T>(rs, voff[r]);
+    w0 = plane_bits(v0);
+  }
+  // two register sets: the next group's loads are in flight while the
+  // current group is classified
+  T va[ZU][TY + 1], vb[ZU][TY + 1];
+  int k = zs;
+  if (k < ze) load_group(k, va);
+  while (k < ze) {
+    if (k + ZU < ze) load_group(k + ZU, vb);
+    __builtin_amdgcn_sched_barrier(0);
+    use_group(k, va);
+    k += ZU;
+    if (k >= ze) break;
+    if (k + ZU < ze) load_group(k + ZU, va);
+    __builtin_amdgcn_sched_barrier(0);
+    use_group(k, vb);
+    k += ZU;
+  }
+  if (!xvalid) return;
+#pragma unroll
+  for (int r = 0; r < TY; ++r) {
+    const int j = yb + r;
+    if (j < g.ny - 1) cnt[((int64_t)(xb + lane) * (g.ny - 1) + j) * g.nzseg + s] = acc[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Emit, Marching Cubes.
+//
+// A block owns 63 cell rows along x at one y (samples x in [ib, ib+64),
+// y in {jb, jb+1}) and walks its z segment in chunks of ZC cells.  Per
+// chunk:
+//  * the brick (ZC+1 planes x 2 rows x 64 samples) arrives by LDS DMA, one
+//    buffer_load...lds per sample row (addresses: a scalar base per row plus
+//    lane*4), double-buffered so the next chunk is in flight meanwhile;
+//  * one ballot per sample row turns it into a 64-bit below-iso mask;
+//  * each wave step takes R = 64/ZC rows: lane (h, z) is cell (ci = R*step
+//    + h, jb, zc + z).  The case comes from 4 mask reads, the row's output
+//    slots from a segmented DPP scan (the reference's emission order inside
+//    a row), and active cells are appended to the wave's list with their
+//    slot prefix;
+//  * the list is emitted slot-parallel: lane o takes vertex slot o (and face
+//    slot o), finds its cell by binary search over the list prefixes, and
+//    writes output rbv[row] + slot -- consecutive lanes write consecutive
+//    outputs of a row.
+template <typename T, typename M, typename Out, int ZC, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_emit_mc4(EmitArgs a, Out* __restrict__ verts,
+                                                     int64_t* __restrict__ faces) {
+  constexpr int TX = kEmitTX, SX = 64, SY = 2;
+  constexpr int R = kWave / ZC;
+  constexpr int NP = ZC + 1;
+  constexpr int DW = (int)(sizeof(T) / 4);
+  constexpr int RD = SX * DW;  // dwords per sample row
+  constexpr int NSTEP = (TX + R - 1) / R;
+  constexpr int CAP = 128;
+  constexpr int VW = CAP * 12 / 64, FW = CAP * 5 / 64;  // slot bitmap words
+  struct Smem {
+    uint32_t smp[2][NP * SY * RD];
+    double cz[2][NP];
+    double cx[SX], cy[SY];
+    uint64_t mask[NP * SY];  // [plane][y], bit x
+    uint32_t cnt[256];
+    uint64_t lv[256], lt[256];
+    uint8_t le[16];
+    int64_t rbv[2][SX], rbf[2][SX];
+    uint64_t list[kWaves][CAP];
+    uint16_t lpv[kWaves][CAP], lpf[kWaves][CAP];
+    uint64_t vbits[kWaves][VW], fbits[kWaves][FW];  // bit o: slot o starts an entry
+  };
+  __shared__ Smem sm;
+  const T* __restrict__ f = (const T*)a.f;
+  const Geom g = a.g;
+  const int lane = lane_id(), wv = wave_id();
+  const int64_t t = xcd_tile(blockIdx.x, a.ntiles);
+  const int jb = (int)(t % a.ntj);
+  const int64_t rest = t / a.ntj;
+  const int ib = (int)(rest % a.nti) * TX;
+  const int s = (int)(rest / a.nti);
+  const int zs = s * g.zseg, ze = min(zs + g.zseg, g.nz - 1);
+  const T thr = (T)a.thr;
+  const M iso = (M)a.iso;
+  const bool cx32 = a.cxmask & 1, cy32 = (a.cxmask >> 1) & 1, cz32 = (a.cxmask >> 2) & 1;
+  const double* zp = a.coords + g.nx + g.ny;
+  const int64_t totv = a.totals[0], totf = a.totals[1];
+  const int64_t nelem = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t pstride = (int64_t)g.nx * g.ny;
+
+  for (int c = threadIdx.x; c < 256; c += kBlock) {
+    const uint32_t m = nat_to_mc((uint32_t)c);
+    const uint64_t vw = MC_CASE_VERTS[m], tw = MC_CASE_TRIS[m];
+    sm.cnt[c] = (uint32_t)(vw & 0xF) | ((uint32_t)(tw & 0x7) << 16);
+    sm.lv[c] = vw;
+    sm.lt[c] = tw;
+  }
+  if (threadIdx.x < 12) sm.le[threadIdx.x] = MC_EDGE[threadIdx.x];
+  for (int c = threadIdx.x; c < SX; c += kBlock) {
+    int64_t bv = 0, bf = 0;
+    if (c < TX && ib + c < g.nx - 1) {
+      const int64_t u = ((int64_t)(ib + c) * (g.ny - 1) + jb) * g.nzseg + s;
+      bv = a.voff[u];
+      bf = a.foff[u];
+    }
+    sm.rbv[0][c] = bv;
+    sm.rbf[0][c] = bf;
+    sm.cx[c] = a.coords[min(ib + c, g.nx - 1)];
+  }
+  if (threadIdx.x < SY) sm.cy[threadIdx.x] = a.coords[g.nx + min(jb + (int)threadIdx.x, g.ny - 1)];
+
+  // DMA layout.  VEC: 16 bytes per lane, so one instruction fills 1 KiB of
+  // LDS = 1024/(4*RD) whole sample rows; the per-lane byte offsets relative
+  // to the chunk's first sample are the same for every chunk and are
+  // computed once.  Otherwise one dword per lane, one instruction per
+  // 64-dword piece of a row.
+  constexpr int NROWS = NP * SY;
+  constexpr int RPI = VEC ? 1024 / (4 * RD) : 1;
+  constexpr int NI = VEC ? (NROWS + RPI - 1) / RPI : NROWS * DW;
+  constexpr int NIW = (NI + kWaves - 1) / kWaves;
+  uint32_t doff[NIW];
+  bool dok[NIW];
+#pragma unroll
+  for (int m = 0; m < NIW; ++m) {
+    const int ins = wv + m * kWaves;
+    int row, byte_in_row;
+    if (VEC) {
+      row = ins * RPI + lane / (kWave / RPI);
+      byte_in_row = (lane % (kWave / RPI)) * 16;
+    } else {
+      row = ins / DW;
+      byte_in_row = (ins % DW) * 256 + lane * 4;
+    }
+    dok[m] = ins < NI && row < NROWS;
+    const int pl = row / SY, y = row - pl * SY;
+    doff[m] = (uint32_t)(((int64_t)pl * pstride + (int64_t)y * g.nx) * (int64_t)sizeof(T)) +
+              (uint32_t)byte_in_row;
+  }
+  auto fetch = [&](int zc, int buf) {
+    const int64_t start = (int64_t)zc * pstride + (int64_t)jb * g.nx + ib;
+    const int64_t left = (nelem - start) * (int64_t)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs =
+        buffer_rsrc(f + start, (uint32_t)(left < 0x7FFFFFFF ? left : 0x7FFFFFFF));
+#pragma unroll
+    for (int m = 0; m < NIW; ++m) {
+      const int ins = wv + m * kWaves;
+      if (dok[m]) dma_to_lds<VEC ? 16 : 4>(rs, sm.smp[buf] + ins * (VEC ? 256 : kWave), doff[m]);
+    }
+    constexpr int NZ = (2 * NP + kWave - 1) / kWave;
+    uint32_t* zd = (uint32_t*)sm.cz[buf];
+    for (int ins = wv; ins < NZ; ins += kWaves) {
+      const int p = ins * kWave + lane;
+      if (p < 2 * NP) {
+        const double* src = zp + min(zc + (p >> 1), g.nz - 1);
+        __builtin_amdgcn_global_load_lds((const void*)((const char*)src + 4 * (p & 1)),
+                                         (void*)(zd + ins * kWave), 4, 0, 0);
+      }
+    }
+  };
+
+  uint64_t* list = sm.list[wv];
+  uint16_t* lpv = sm.lpv[wv];
+  uint16_t* lpf = sm.lpf[wv];
+  uint64_t* vbits = sm.vbits[wv];
+  uint64_t* fbits = sm.fbits[wv];
+  for (int q = lane; q < VW; q += kWave) vbits[q] = 0;
+  for (int q = lane; q < FW; q += kWave) fbits[q] = 0;
+  const int h = lane / ZC, z = lane % ZC;
+
+  if (zs < ze) fetch(zs, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  lds_barrier();
+  int buf = 0;
+  for (int zc = zs; zc < ze; zc += ZC, buf ^= 1) {
+    if (zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
+    const uint32_t* smp = sm.smp[buf];
+    const double* czs = sm.cz[buf];
+    const int64_t* rbv = sm.rbv[buf];
+    const int64_t* rbf = sm.rbf[buf];
+    int64_t* rbv_next = sm.rbv[buf ^ 1];
+    int64_t* rbf_next = sm.rbf[buf ^ 1];
+
+    // below-iso masks of this chunk's sample rows
+    for (int row = wv; row < NP * SY; row += kWaves) {
+      const T v = *(const T*)(smp + row * RD + lane * DW);
+      const uint64_t b = __ballot(v < thr);
+      if (lane == 0) sm.mask[row] = b;
+    }
+    lds_barrier();
+
+    auto sample = [&](int c, int ci, int corner) -> T {
+      return *(const T*)(smp + ((c + mc_dz(corner)) * SY + mc_dy(corner)) * RD +
+                         (ci + mc_dx(corner)) * DW);
+    };
+    int n = 0;
+    uint32_t V = 0, F = 0;
+    auto flush = [&]() {
+      // slot o belongs to the last entry starting at or before o: count
+      // the start bits up to o
+      int before = 0;
+      for (uint32_t o0 = 0; o0 < V; o0 += kWave) {
+        const uint32_t o = o0 + lane;
+        const uint64_t word = vbits[o0 >> 6];
+        const uint64_t upto = word & (~0ull >> (kWave - 1 - lane));
+        const int k = before + __popcll(upto) - 1;
+        before += __popcll(word);
+        if (o >= V) break;
+        const uint64_t e = list[k];
+        const int ci = (int)(e & 63), c = (int)((e >> 6) & 31);
+        const uint32_t cs = (uint32_t)((e >> 11) & 0xFF), pv = (uint32_t)((e >> 19) & 0x1FF);
+        const uint32_t q = o - lpv[k];
+        const uint32_t ed = (uint32_t)(sm.lv[cs] >> (4 + 4 * q)) & 0xF;
+        const uint32_t ew = sm.le[ed];
+        const int ca = ew & 0xF, cb = ew >> 4;
+        const T va = sample(c, ci, ca), vb = sample(c, ci, cb);
+        const M mu = (iso - (M)va) / (M)(vb - va);
+        const double px = lerp_axis<M>(cx32, mu, sm.cx[ci + mc_dx(ca)], sm.cx[ci + mc_dx(cb)]);
+        const double py = lerp_axis<M>(cy32, mu, sm.cy[mc_dy(ca)], sm.cy[mc_dy(cb)]);
+        const double pz = lerp_axis<M>(cz32, mu, czs[c + mc_dz(ca)], czs[c + mc_dz(cb)]);
+        const int64_t vi = rbv[ci] + pv + q;
+        if (vi < totv) {
+          Out* out = verts + vi * 3;
+          out[0] = (Out)px;
+          out[1] = (Out)py;
+          out[2] = (Out)pz;
+        }
+      }
+      before = 0;
+      for (uint32_t o0 = 0; o0 < F; o0 += kWave) {
+        const uint32_t o = o0 + lane;
+        const uint64_t word = fbits[o0 >> 6];
+        const uint64_t upto = word & (~0ull >> (kWave - 1 - lane));
+        const int k = before + __popcll(upto) - 1;
+        before += __popcll(word);
+        if (o >= F) break;
+        const uint64_t e = list[k];
+        const int ci = (int)(e & 63);
+        const uint32_t cs = (uint32_t)((e >> 11) & 0xFF), pv = (uint32_t)((e >> 19) & 0x1FF),
+                       pf = (uint32_t)((e >> 28) & 0x1FF);
+        const uint32_t q = o - lpf[k];
+        const uint64_t w = sm.lt[cs] >> (3 + 12 * q);
+        const int64_t vb = rbv[ci] + pv + 1;  // 1-based
+        const int64_t fi = rbf[ci] + pf + q;
+        if (fi < totf) {
+          int64_t* out = faces + fi * 3;
+          out[0] = vb + (int64_t)(w & 0xF);
+          out[1] = vb + (int64_t)((w >> 4) & 0xF);
+          out[2] = vb + (int64_t)((w >> 8) & 0xF);
+        }
+      }
+      wave_sync();
+      for (uint32_t q = lane; q < (V + 63) / 64; q += kWave) vbits[q] = 0;
+      for (uint32_t q = lane; q < (F + 63) / 64; q += kWave) fbits[q] = 0;
+    };
+
+    const int kc = zc + z;
+    const bool zvalid = kc < ze;
+    for (int st = wv; st < NSTEP; st += kWaves) {
+      const int ci = st * R + h;
+      const bool valid = zvalid && ci < TX && ib + ci < g.nx - 1;
+      const uint64_t* mz = sm.mask + z * SY;
+      const uint32_t lo = (uint32_t)((mz[0] >> ci) & 3) | ((uint32_t)((mz[1] >> ci) & 3) << 2);
+      const uint32_t hi =
+          (uint32_t)((mz[SY] >> ci) & 3) | ((uint32_t)((mz[SY + 1] >> ci) & 3) << 2);
+      const uint32_t cs = valid ? (lo | (hi << 4)) : 0u;
+      const uint32_t packed = sm.cnt[cs];
+      const uint32_t inc = seg_scan_dpp<ZC>(packed);
+      uint32_t tots[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        tots[r] = (uint32_t)__builtin_amdgcn_readlane((int)inc, r * ZC + ZC - 1);
+      uint32_t before = 0, mine = tots[0], sum = 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r < h) before += tots[r];
+        if (r == h) mine = tots[r];
+        sum += tots[r];
+      }
+      if (z == 0 && ci < SX) {
+        rbv_next[ci] = rbv[ci] + (mine & 0xFFFF);
+        rbf_next[ci] = rbf[ci] + (mine >> 16);
+      }
+      if (sum == 0) continue;
+      const uint32_t ex_row = inc - packed;   // prefix inside the cell's row
+      const uint32_t ex = ex_row + before;    // prefix inside the step
+      const bool active = packed != 0;
+      const uint64_t bal = __ballot(active);
+      const int na = __popcll(bal);
+      if (n + na > CAP) {
+        wave_sync();
+        flush();
+        wave_sync();
+        n = 0;
+        V = 0;
+        F = 0;
+      }
+      if (active) {
+        const int k = n + (int)lanes_below(bal);
+        list[k] = (uint64_t)ci | ((uint64_t)z << 6) | ((uint64_t)cs << 11) |
+                  ((uint64_t)(ex_row & 0xFFFF) << 19) | ((uint64_t)(ex_row >> 16) << 28);
+        const uint32_t sv = V + (ex & 0xFFFF), sf = F + (ex >> 16);
+        lpv[k] = (uint16_t)sv;
+        lpf[k] = (uint16_t)sf;
+        atomicOr((unsigned long long*)&vbits[sv >> 6], 1ull << (sv & 63));
+        atomicOr((unsigned long long*)&fbits[sf >> 6], 1ull << (sf & 63));
+      }
+      n += na;
+      V += sum & 0xFFFF;
+      F += sum >> 16;
+    }
+    // the next brick must land before the barrier that ends this chunk; wait
+    // for it before this chunk's last stores are issued
+    __builtin_amdgcn_s_waitcnt(0);
+    wave_sync();
+    flush();
+    lds_barrier();
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+template <typename T, int METHOD>
+static hipError_t count4_t(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
+                           hipStream_t st) {
+  constexpr int TY = sizeof(T) == 4 ? 8 : 6, ZU = 4;
+  const int nbx = (g.nx - 1 + kWaves * (kWave - 1) - 1) / (kWaves * (kWave - 1));
+  const int nby = (g.ny - 1 + TY - 1) / TY;
+  const int64_t nb = (int64_t)nbx * nby * g.nzseg;
+  hipLaunchKernelGGL((k_count4<T, METHOD, TY, ZU>), dim3((unsigned)nb), dim3(kBlock), 0, st,
+                     (const T*)field, g, (T)p.thr, cnt, nbx, nby, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_count4(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
+                         hipStream_t st) {
+  if (p.T == K_F32) {
+    if (p.method == MC) return count4_t<float, MC>(p, g, field, cnt, st);
+    if (p.method == MT) return count4_t<float, MT>(p, g, field, cnt, st);
+    return count4_t<float, SN>(p, g, field, cnt, st);
+  }
+  if (p.method == MC) return count4_t<double, MC>(p, g, field, cnt, st);
+  if (p.method == MT) return count4_t<double, MT>(p, g, field, cnt, st);
+  return count4_t<double, SN>(p, g, field, cnt, st);
+}
+
+template <typename T, typename M, typename Out>
+static hipError_t emit_mc4_t(const EmitArgs& a, void* verts, int64_t* faces, hipStream_t st) {
+  constexpr int ZC = sizeof(T) == 4 ? 32 : 16;
+  // the brick's DMA offsets are 32-bit and span ZC+1 planes
+  if ((int64_t)(ZC + 1) * a.g.nx * a.g.ny * (int64_t)sizeof(T) >= (1ll << 31))
+    return hipErrorNotSupported;
+  EmitArgs b = a;
+  b.nti = (a.g.nx - 1 + kEmitTX - 1) / kEmitTX;
+  b.ntj = a.g.ny - 1;
+  b.ntiles = (int64_t)b.nti * b.ntj * a.g.nzseg;
+  // 16-byte DMA needs 16-byte aligned sample rows
+  const bool vec = ((uintptr_t)a.f % 16 == 0) && (((int64_t)a.g.nx * (int64_t)sizeof(T)) % 16 == 0);
+  if (vec)
+    hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, true>), dim3((unsigned)b.ntiles), dim3(kBlock),
+                       0, st, b, (Out*)verts, faces);
+  else
+    hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, false>), dim3((unsigned)b.ntiles), dim3(kBlock),
+                       0, st, b, (Out*)verts, faces);
+  return hipGetLastError();
+}
+
+hipError_t launch_emit_mc4(const Plan& p, const EmitArgs& a, void* verts, int64_t* faces,
+                           hipStream_t st) {
+  if (p.T == K_F32 && p.M == K_F32 && p.Out == K_F32)
+    return emit_mc4_t<float, float, float>(a, verts, faces, st);
+  if (p.T == K_F32 && p.M == K_F32) return emit_mc4_t<float, float, double>(a, verts, faces, st);
+  if (p.T == K_F32) return emit_mc4_t<float, double, double>(a, verts, faces, st);
+  return emit_mc4_t<double, double, double>(a, verts, faces, st);
+}
+
+}  // namespace msh
