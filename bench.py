@@ -13,9 +13,14 @@ MarchingCubes(iso=0), X=Y=Z=-1:1.  The same field in Float32 is measured in
 the same run and reported under "float32".
 
 Printed: one JSON line.  `value` is Gvoxels/s (samples per second / 1e9).
-`roofline` is for the dominant kernel (the emit kernel), from HIP events on
-the stream the library launches on.  `cpu_baseline` times the CPU oracle
-(single thread) on an x-slab of the same field.
+`roofline` is for the dominant kernel (the Marching Cubes emit kernel,
+k_emit_mc4), from HIP events on the stream the library launches on; its
+`traffic` is the HBM bytes per launch from the rocprofv3 FETCH_SIZE and
+WRITE_SIZE summaries kept in profiles/ (null when absent).  `cpu_baseline`
+times the CPU oracle (single thread) on an x-slab of the same field.  A
+1 GiB device-to-device copy is timed at the end as the practical HBM rate
+of the same box (and is the known-size kernel the counter scale factor is
+read from).
 """
 import argparse
 import ctypes
@@ -39,7 +44,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1024, help="cube edge")
     ap.add_argument("--method", default="mc", choices=["mc", "mt", "sn"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-planes", type=int, default=96,
+    ap.add_argument("--cpu-planes", type=int, default=512,
                     help="x-planes of the field the CPU baseline meshes")
     return ap.parse_args()
 
@@ -131,6 +136,35 @@ def measure(m, dtype, n, method_name, steps, warmup):
     return res
 
 
+def copy_probe():
+    """Time a 1 GiB device-to-device tensor copy (2 GiB of HBM traffic)."""
+    import torch
+    a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    best = None
+    for _ in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3
+        best = t if best is None else min(best, t)
+    del a, b
+    torch.cuda.empty_cache()
+    return dict(bytes=2 * 1024 ** 3, best_ms=round(best * 1e3, 4),
+                GBps=round(2 * 1024 ** 3 / best / 1e9, 1))
+
+
+def traffic_of(method, n, dtype):
+    """HBM bytes per emit launch from the committed rocprofv3 counter summary."""
+    tfile = ROOT / "profiles" / f"traffic_{method}_value_noise_{n}.json"
+    if not tfile.exists():
+        return None
+    return json.loads(tfile.read_text()).get(dtype, {}).get("bytes_per_launch")
+
+
 def cpu_baseline(m, n, planes):
     """The CPU oracle (single-threaded restatement of the reference loop) on
     the first `planes` x-planes of the same Float64 field."""
@@ -165,12 +199,18 @@ def main():
     f64 = measure(m, "float64", a.n, a.method, a.steps, a.warmup)
     f32 = measure(m, "float32", a.n, a.method, a.steps, a.warmup)
     main_r = f64
-    ach = (main_r["field_bytes"] + main_r["out_bytes"]) / main_r["emit_s"]
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{a.method}_value_noise_{a.n}_f64.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    copy = copy_probe()
     cpu = None if a.no_cpu else cpu_baseline(m, a.n, a.cpu_planes)
+    kname = {"mc": "k_emit_mc4", "mt": "k_emit (tetrahedra)", "sn": "k_emit (surface nets)"}[a.method]
+
+    def roofline(r):
+        b = r["field_bytes"] + r["out_bytes"]
+        ach = b / r["emit_s"]
+        return {"bound": "hbm", "achieved": round(ach / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4),
+                "traffic": traffic_of(a.method, a.n, r["dtype"]),
+                "kernel": kname, "algorithmic_bytes_per_launch": b,
+                "avg_launch_ms": round(r["emit_s"] * 1e3, 4)}
 
     def summary(r):
         b = r["field_bytes"] + r["out_bytes"]
@@ -179,7 +219,7 @@ def main():
                     count_ms=round(r["count_s"] * 1e3, 4), emit_ms=round(r["emit_s"] * 1e3, 4),
                     nverts=r["nverts"], nfaces=r["nfaces"], algorithmic_bytes=b,
                     step_hbm_frac=round(b / r["step_s"] / HBM_PEAK, 4),
-                    emit_hbm_frac=round(b / r["emit_s"] / HBM_PEAK, 4))
+                    roofline=roofline(r))
 
     out = {
         "metric": "Gvoxels/s + % HBM roofline (1024^3 MC)",
@@ -198,11 +238,9 @@ def main():
                                f"Float64 {a.n}^3 value noise (32^3 lattice + 0.25 x 128^3 "
                                "octave, seed 1), X=Y=Z=-1:1",
                    "shape": [a.n] * 3, "seed": 1},
-        "roofline": {"bound": "hbm", "achieved": round(ach / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4), "traffic": traffic,
-                     "kernel": "k_emit_mc (emit pass)",
-                     "algorithmic_bytes_per_launch": main_r["field_bytes"] + main_r["out_bytes"]},
+        "roofline": roofline(main_r),
         "cpu_baseline": cpu,
+        "hbm_copy_probe": copy,
         "float64": summary(f64),
         "float32": summary(f32),
     }
