@@ -46,6 +46,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-planes", type=int, default=512,
                     help="x-planes of the field the CPU baseline meshes")
+    ap.add_argument("--batch-volumes", type=int, default=512,
+                    help="volumes of the batched-vs-plain measurement (0 skips it)")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests of the serving measurement (0 skips it)")
+    ap.add_argument("--serve-qps", default="100,200,400,800,0",
+                    help="offered request rates of the serving sweep (0: all at once)")
     return ap.parse_args()
 
 
@@ -136,6 +142,179 @@ def measure(m, dtype, n, method_name, steps, warmup):
     return res
 
 
+class BatchRunner:
+    """The batched form through the C ABI, outputs preallocated per volume."""
+
+    def __init__(self, m, fields, methods):
+        import torch
+        from importlib import import_module
+        self.capi = capi = import_module(m.__name__ + "._capi")
+        self.L = L = capi.lib()
+        n = self.n = len(fields)
+        self.fields = fields
+        self.F = (capi.MshField * n)(*[m._field(f)[0] for f in fields])
+        self.P = (capi.MshParams * n)(*[m._params(me, (-1, 1), (-1, 1), (-1, 1))
+                                         for me in methods])
+        self.ctx = capi.context(fields[0].device.index)
+        self.ctx.check(L.msh_set_stream(self.ctx.handle,
+                                        ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        self.nv = (ctypes.c_int64 * n)()
+        self.nf = (ctypes.c_int64 * n)()
+        self.ctx.check(L.msh_batch_count(self.ctx.handle, n, self.F, self.P, self.nv, self.nf))
+        self.vp = (ctypes.c_void_p * n)()
+        self.fp = (ctypes.c_void_p * n)()
+        self.keep = []
+        for v in range(n):
+            ok = ctypes.c_int32()
+            self.ctx.check(L.msh_output_kind(ctypes.byref(self.F[v]), ctypes.byref(self.P[v]),
+                                             ctypes.byref(ok)))
+            dt = torch.float32 if ok.value == capi.MSH_FLOAT32 else torch.float64
+            V = torch.empty((max(self.nv[v], 1), 3), dtype=dt, device=fields[v].device)
+            Fv = torch.empty((max(self.nf[v], 1), 3), dtype=torch.int64, device=fields[v].device)
+            self.keep += [V, Fv]
+            self.vp[v], self.fp[v] = V.data_ptr(), Fv.data_ptr()
+
+    def step(self):
+        L, c = self.L, self.ctx
+        c.check(L.msh_batch_count(c.handle, self.n, self.F, self.P, self.nv, self.nf))
+        c.check(L.msh_batch_extract(c.handle, self.n, self.F, self.P, self.vp, self.fp,
+                                    self.capi.MSH_DEVICE))
+
+
+def batch_workload(m, nvol, seed):
+    """Mixed-size batch: axes iid Uniform{32..256}; field a seeded choice of
+    sphere / torus (R=0.6, r=0.2) / gyroid / 16-Gaussian blobs; method a
+    seeded choice of MarchingCubes / MarchingTetrahedra / NaiveSurfaceNets,
+    iso=0; Float32 fields."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    kinds = ["sphere", "torus", "gyroid", "blobs"]
+    meths = [m.MarchingCubes(iso=0), m.MarchingTetrahedra(iso=0), m.NaiveSurfaceNets(iso=0)]
+    fields, methods = [], []
+    for v in range(nvol):
+        shape = tuple(int(x) for x in rng.integers(32, 257, size=3))
+        fields.append(m.generate_field(kinds[int(rng.integers(4))], shape, "float32",
+                                       seed=seed * 100003 + v))
+        methods.append(meths[int(rng.integers(3))])
+    return fields, methods
+
+
+def measure_batch(m, nvol, steps, seed=7):
+    """Plain (one count + extract per volume, back to back) against batched
+    (one msh_batch_count + msh_batch_extract for all volumes)."""
+    import torch
+    fields, methods = batch_workload(m, nvol, seed)
+    torch.cuda.synchronize()
+    plain = [Runner(m, f, me) for f, me in zip(fields, methods)]
+    br = BatchRunner(m, fields, methods)
+    for v in range(nvol):
+        assert (plain[v].nv, plain[v].nf) == (br.nv[v], br.nf[v])
+    samples = sum(f.numel() for f in fields)
+    # warm-up
+    for r in plain:
+        r.step()
+    br.step()
+    torch.cuda.synchronize()
+    t_plain = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        for r in plain:
+            r.step()
+        torch.cuda.synchronize()
+        t_plain.append(time.perf_counter() - t0)
+    t_batch = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        br.step()
+        torch.cuda.synchronize()
+        t_batch.append(time.perf_counter() - t0)
+    tp, tb = min(t_plain), min(t_batch)
+    res = dict(volumes=nvol, samples=samples, seed=seed,
+               faces=sum(br.nf[v] for v in range(nvol)),
+               plain_ms=round(tp * 1e3, 3), batched_ms=round(tb * 1e3, 3),
+               plain_req_s=round(nvol / tp, 1), batched_req_s=round(nvol / tb, 1),
+               batched_over_plain=round(tp / tb, 3),
+               plain_gvoxels_s=round(samples / tp / 1e9, 3),
+               batched_gvoxels_s=round(samples / tb / 1e9, 3))
+    del plain, br, fields
+    torch.cuda.empty_cache()
+    return res
+
+
+def serve_workload(m, nreq, seed):
+    """Serving requests: value noise (as the 1024^3 field, at the request's
+    size), cubic size log-uniform in 64^3..512^3, Float32 80% / Float64 20%,
+    method a seeded choice of MC / MT / SN with iso=0.  Fields are generated
+    and resident before the run."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    meths = [m.MarchingCubes(iso=0), m.MarchingTetrahedra(iso=0), m.NaiveSurfaceNets(iso=0)]
+    out = []
+    for v in range(nreq):
+        n = int(round(64 * 8 ** rng.random()))
+        dt = "float64" if rng.random() < 0.2 else "float32"
+        f = m.generate_field("value_noise", (n, n, n), dt, seed=seed * 100003 + v)
+        out.append((f, meths[int(rng.integers(3))]))
+    return out
+
+
+def measure_serve(m, nreq, qps_list, seed=11):
+    """Plain sequential extraction of the request set, then the continuous-
+    batching server under Poisson arrivals at each offered rate."""
+    import numpy as np
+    import torch
+    reqs = serve_workload(m, nreq, seed)
+    torch.cuda.synchronize()
+    samples = sum(f.numel() for f, _ in reqs)
+    plain = [Runner(m, f, me) for f, me in reqs]
+    for r in plain:
+        r.step()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for r in plain:
+        t1 = time.perf_counter()
+        r.step()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+    tp = time.perf_counter() - t0
+    del plain
+    res = dict(requests=nreq, samples=samples, seed=seed,
+               plain=dict(req_s=round(nreq / tp, 1), p50_ms=round(np.percentile(lat, 50) * 1e3, 3),
+                          p99_ms=round(np.percentile(lat, 99) * 1e3, 3)),
+               served=[])
+    rng = np.random.default_rng(seed + 1)
+    for qps in qps_list:
+        gaps = rng.exponential(1.0 / qps, size=nreq) if qps else np.zeros(nreq)
+        with m.Server(hbm_budget=64 << 30, max_batch=128) as srv:
+            # warm the server's contexts and workspaces once
+            srv.submit(*reqs[0]).result()
+            futs = []
+            t0 = time.perf_counter()
+            due = t0
+            for (f, me), gap in zip(reqs, gaps):
+                due += gap
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                futs.append(srv.submit(f, me))
+            for fu in futs:
+                fu.result()
+            span = time.perf_counter() - t0
+            lats = [fu.latency for fu in futs]
+            st = dict(srv.stats)
+        res["served"].append(dict(offered_qps=qps or "all at t=0", req_s=round(nreq / span, 1),
+                                  p50_ms=round(np.percentile(lats, 50) * 1e3, 3),
+                                  p99_ms=round(np.percentile(lats, 99) * 1e3, 3),
+                                  batches=st["batches"] - 1,
+                                  mean_batch=round((st["requests"] - 1) / max(st["batches"] - 1, 1), 2)))
+    best = max(x["req_s"] for x in res["served"])
+    res["served_over_plain"] = round(best / res["plain"]["req_s"], 3)
+    del reqs
+    torch.cuda.empty_cache()
+    return res
+
+
 def copy_probe():
     """Time a 1 GiB device-to-device tensor copy (2 GiB of HBM traffic)."""
     import torch
@@ -199,6 +378,9 @@ def main():
     f64 = measure(m, "float64", a.n, a.method, a.steps, a.warmup)
     f32 = measure(m, "float32", a.n, a.method, a.steps, a.warmup)
     main_r = f64
+    batch = measure_batch(m, a.batch_volumes, max(2, min(a.steps, 5))) if a.batch_volumes else None
+    serve = (measure_serve(m, a.serve_requests, [float(q) for q in a.serve_qps.split(",")])
+             if a.serve_requests else None)
     copy = copy_probe()
     cpu = None if a.no_cpu else cpu_baseline(m, a.n, a.cpu_planes)
     kname = {"mc": "k_emit_mc4", "mt": "k_emit (tetrahedra)", "sn": "k_emit (surface nets)"}[a.method]
@@ -241,6 +423,8 @@ def main():
         "roofline": roofline(main_r),
         "cpu_baseline": cpu,
         "hbm_copy_probe": copy,
+        "batched": batch,
+        "serving": serve,
         "float64": summary(f64),
         "float32": summary(f32),
     }

@@ -95,6 +95,21 @@ int msh_count(void* ctx, const msh_field* field, const msh_params* params,
 int msh_extract(void* ctx, const msh_field* field, const msh_params* params,
                 void* verts, int64_t* faces, int32_t out_memory);
 
+/* Batched form: n independent volumes (any sizes, element types, methods
+ * and parameters) through one classify-count-scan sequence and one emit
+ * launch per kernel instantiation, instead of n separate calls.  Element v of
+ * the results is what msh_count / msh_extract return for fields[v] and
+ * params[v] alone (same reference functions as those two).  msh_batch_count
+ * writes nverts[v] and nfaces[v]; msh_batch_extract, called next with the same
+ * arrays, writes volume v into verts[v] and faces[v] (faces 1-based within
+ * the volume).  Fields may be host or device memory; outputs as out_memory
+ * says.  A batch shares the context's workspace with the single-volume calls:
+ * msh_count in between invalidates it (MSH_ERR_STATE). */
+int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
+                    int64_t* nverts, int64_t* nfaces);
+int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
+                      void* const* verts, int64_t* const* faces, int32_t out_memory);
+
 /* Synthetic fields, generated on the device into a column-major buffer of
  * nx*ny*nz elements of `dtype`.  kind: 0 sphere sqrt(x^2+y^2+z^2)-p0 on
  * [-1,1]^3; 1 torus (R=p0, r=p1) on [-1,1]^3; 2 gyroid on [-p0,p0]^3;

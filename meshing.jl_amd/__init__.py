@@ -32,8 +32,9 @@ from . import _capi
 from ._capi import (MSH_DEVICE, MSH_FLOAT32, MSH_FLOAT64, MSH_HOST, MSH_INT64, MshField,
                     MshParams, context)
 
-__all__ = ["isosurface", "AbstractMeshingAlgorithm", "MarchingCubes", "MarchingTetrahedra",
-           "NaiveSurfaceNets", "device_field", "generate_field", "count"]
+__all__ = ["isosurface", "isosurface_batch", "AbstractMeshingAlgorithm", "MarchingCubes",
+           "MarchingTetrahedra", "NaiveSurfaceNets", "device_field", "generate_field", "count",
+           "Server", "FieldSpec"]
 
 
 class AbstractMeshingAlgorithm:
@@ -243,6 +244,94 @@ def isosurface(sdf, method=None, *args, X=(-1, 1), Y=(-1, 1), Z=(-1, 1), device=
     return V, F
 
 
+def isosurface_batch(fields, methods=None, axes=None, device=0):
+    """Batched form of :func:`isosurface`: element v of the result is
+    ``isosurface(fields[v], methods[v], *axes[v])``, computed for all volumes
+    together (one count/scan pass and one emit launch per kernel
+    instantiation instead of one call each).
+
+    ``methods`` is one algorithm object for all volumes or a list (default
+    ``MarchingCubes()``); ``axes`` is ``None`` (``-1:1`` on every axis) or a
+    list of ``(X, Y, Z)`` triples (``None`` entries allowed).  Fields are all
+    host arrays or all CUDA tensors on one device; device fields give device
+    results, which are views into one vertex buffer per output type and one
+    face buffer."""
+    n = len(fields)
+    if methods is None or isinstance(methods, AbstractMeshingAlgorithm):
+        methods = [methods or MarchingCubes()] * n
+    if axes is None:
+        axes = [None] * n
+    if len(methods) != n or len(axes) != n:
+        raise ValueError("fields, methods and axes must have the same length")
+    for m in methods:
+        if not isinstance(m, AbstractMeshingAlgorithm) or m.method < 0:
+            raise TypeError(f"no isosurface method for {type(m).__name__}")
+    if n == 0:
+        return []
+    prepared = [_field(f) for f in fields]
+    on_dev = {d for _, _, d in prepared}
+    if len(on_dev) != 1:
+        raise ValueError("fields must be all host arrays or all CUDA tensors")
+    on_dev = on_dev.pop()
+    if on_dev and len({k.device for _, k, _ in prepared}) != 1:
+        raise ValueError("device fields must be on one device")
+    F_arr = (MshField * n)(*[f for f, _, _ in prepared])
+    P_arr = (MshParams * n)(*[_params(m, *(ax if ax is not None else ((-1, 1),) * 3))
+                             for m, ax in zip(methods, axes)])
+    L = _capi.lib()
+    kinds = []
+    for v in range(n):
+        ok = ctypes.c_int32()
+        _capi.check_status(L.msh_output_kind(ctypes.byref(F_arr[v]), ctypes.byref(P_arr[v]),
+                                             ctypes.byref(ok)), f"volume {v}")
+        kinds.append(ok.value)
+    dev = prepared[0][1].device if on_dev else None
+    ctx = context(dev.index if on_dev else device)
+    nv = (ctypes.c_int64 * n)()
+    nf = (ctypes.c_int64 * n)()
+    out = []
+    with ctx.lock:
+        if on_dev:
+            torch = _torch()
+            stream = torch.cuda.current_stream(dev)
+            ctx.check(L.msh_set_stream(ctx.handle, ctypes.c_void_p(stream.cuda_stream)))
+        else:
+            ctx.check(L.msh_set_stream(ctx.handle, None))
+        ctx.check(L.msh_batch_count(ctx.handle, n, F_arr, P_arr, nv, nf))
+        vp = (ctypes.c_void_p * n)()
+        fp = (ctypes.c_void_p * n)()
+        if on_dev:
+            tot32 = sum(nv[v] for v in range(n) if kinds[v] == MSH_FLOAT32)
+            tot64 = sum(nv[v] for v in range(n) if kinds[v] != MSH_FLOAT32)
+            V32 = torch.empty((tot32, 3), dtype=torch.float32, device=dev)
+            V64 = torch.empty((tot64, 3), dtype=torch.float64, device=dev)
+            FF = torch.empty((sum(nf), 3), dtype=torch.int64, device=dev)
+            o32 = o64 = of = 0
+            for v in range(n):
+                if kinds[v] == MSH_FLOAT32:
+                    V = V32[o32:o32 + nv[v]]
+                    o32 += nv[v]
+                else:
+                    V = V64[o64:o64 + nv[v]]
+                    o64 += nv[v]
+                Fv = FF[of:of + nf[v]]
+                of += nf[v]
+                vp[v] = V.data_ptr() if nv[v] else None
+                fp[v] = Fv.data_ptr() if nf[v] else None
+                out.append((V, Fv))
+            mem = MSH_DEVICE
+        else:
+            for v in range(n):
+                V = np.empty((nv[v], 3), dtype=np.float32 if kinds[v] == MSH_FLOAT32 else np.float64)
+                Fv = np.empty((nf[v], 3), dtype=np.int64)
+                vp[v] = V.ctypes.data if nv[v] else None
+                fp[v] = Fv.ctypes.data if nf[v] else None
+                out.append((V, Fv))
+            mem = MSH_HOST
+        ctx.check(L.msh_batch_extract(ctx.handle, n, F_arr, P_arr, vp, fp, mem))
+    return out
+
+
 FIELD_KINDS = {"sphere": 0, "torus": 1, "gyroid": 2, "value_noise": 3, "blobs": 4}
 
 
@@ -269,3 +358,6 @@ def generate_field(kind, shape, dtype="float32", seed=0, p0=None, p1=None, devic
                                                  seed & 0xFFFFFFFFFFFFFFFF, float(p0), float(p1),
                                                  ctypes.c_void_p(buf.data_ptr())))
     return buf.permute(2, 1, 0)
+
+
+from .serve import FieldSpec, Server  # noqa: E402

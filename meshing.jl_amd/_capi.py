@@ -19,7 +19,7 @@ MSH_HOST, MSH_DEVICE = range(2)
 # every symbol the header declares; checked by the CPU test suite
 EXPORTS = ("msh_create", "msh_destroy", "msh_set_stream", "msh_synchronize",
            "msh_last_error", "msh_output_kind", "msh_count", "msh_extract",
-           "msh_generate_field")
+           "msh_batch_count", "msh_batch_extract", "msh_generate_field")
 
 
 class MshField(ctypes.Structure):
@@ -40,6 +40,17 @@ class MeshingError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(msg)
         self.code = code
+
+
+def check_status(rc, what):
+    """Raise for a status returned by a call that has no context (no message)."""
+    if rc == MSH_OK:
+        return
+    if rc == MSH_ERR_ARGUMENT:
+        raise ValueError(f"{what}: invalid argument")
+    if rc == MSH_ERR_UNSUPPORTED:
+        raise TypeError(f"{what}: unsupported element type")
+    raise MeshingError(rc, f"{what}: status {rc}")
 
 
 _lib = None
@@ -67,6 +78,10 @@ def lib():
             L.msh_output_kind.argtypes = [pf, pp, ctypes.POINTER(i32)]
             L.msh_count.argtypes = [vp, pf, pp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
             L.msh_extract.argtypes = [vp, pf, pp, vp, vp, i32]
+            L.msh_batch_count.argtypes = [vp, i32, pf, pp, ctypes.POINTER(i64),
+                                          ctypes.POINTER(i64)]
+            L.msh_batch_extract.argtypes = [vp, i32, pf, pp, ctypes.POINTER(vp),
+                                            ctypes.POINTER(vp), i32]
             L.msh_generate_field.argtypes = [vp, i32, i32, i64, i64, i64, ctypes.c_uint64,
                                              ctypes.c_double, ctypes.c_double, vp]
             for name in EXPORTS:

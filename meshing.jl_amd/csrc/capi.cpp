@@ -15,6 +15,7 @@
 
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/meshing_hip.h"
 #include "internal.h"
@@ -22,6 +23,26 @@
 using namespace msh;
 
 namespace {
+
+// One kernel launch of a batch: the volumes that share its instantiation.
+struct Launch {
+  Plan plan;                // types (and method) of the group
+  int variant = 0;          // emit variant (emit launches only)
+  int64_t gbase = 0;        // first block in the block -> volume map
+  int64_t nblocks = 0;
+};
+
+// State of the last successful msh_batch_count.
+struct BatchState {
+  bool have = false;
+  std::vector<msh_field> fields;
+  std::vector<msh_params> params;
+  std::vector<Plan> plans;
+  std::vector<VolArgs> vols;       // host copy of the device descriptors
+  std::vector<Launch> counts, emits;
+  std::vector<int64_t> sizes;      // 4 per volume: first vertex, nverts, first face, nfaces
+  int64_t units = 0, ncoords = 0, cblocks = 0, eblocks = 0;
+};
 
 struct Ctx {
   int device = 0;
@@ -35,7 +56,13 @@ struct Ctx {
   size_t fstage_bytes = 0;
   void* ostage = nullptr;
   size_t ostage_bytes = 0;
-  int64_t* pinned = nullptr;  // 2 x int64 for the totals
+  void* bws = nullptr;        // batch: descriptors, block maps, sizes
+  size_t bws_bytes = 0;
+  void* hbatch = nullptr;     // batch: pinned host copy of descriptors and sizes
+  size_t hbatch_bytes = 0;
+  hipEvent_t upload = nullptr;  // last descriptor upload from hbatch
+  bool upload_pending = false;
+  int64_t* pinned = nullptr;  // 4 x int64 for the totals
   // state of the last successful msh_count
   bool have = false;
   msh_field field{};
@@ -43,8 +70,9 @@ struct Ctx {
   Plan plan{};
   Geom geom{};
   Workspace w{};
-  const void* dfield = nullptr;
+  VolArgs vol{};
   int64_t nverts = 0, nfaces = 0;
+  BatchState batch;
 };
 
 int fail(Ctx* c, int code, const std::string& msg) {
@@ -162,7 +190,9 @@ int make_plan(Ctx* c, const msh_field* f, const msh_params* p, Plan* out) {
   return MSH_OK;
 }
 
-Geom make_geom(const msh_field* f) {
+// batch: the launch has blocks enough from the many volumes, so z rows are
+// split only to keep units within 4096 cells
+Geom make_geom(const msh_field* f, bool batch = false) {
   Geom g{};
   g.nx = (int32_t)f->nx;
   g.ny = (int32_t)f->ny;
@@ -182,7 +212,7 @@ Geom make_geom(const msh_field* f) {
   const int64_t blocks_xy = ((cx + 251) / 252) * ((cy + ty - 1) / ty);
   // many long z runs stream better than many short ones: split only when
   // there are fewer than ~2 count blocks per CU
-  int64_t nseg = (512 + blocks_xy - 1) / blocks_xy;
+  int64_t nseg = batch ? 1 : (512 + blocks_xy - 1) / blocks_xy;
   const int64_t max_seg = (cz + 63) / 64;
   if (nseg > max_seg) nseg = max_seg;
   if (nseg < 1) nseg = 1;
@@ -217,15 +247,15 @@ int grow(Ctx* c, void** buf, size_t* have, size_t need, const char* what) {
   return MSH_OK;
 }
 
-int setup_workspace(Ctx* c, const Geom& g, Workspace* w) {
-  const int64_t U = g.units > 0 ? g.units : 1;
+int setup_workspace(Ctx* c, int64_t units, int64_t ncoords, Workspace* w) {
+  const int64_t U = units > 0 ? units : 1;
   const int64_t nt = scan_tiles(U);
   const size_t b_cnt = align256((size_t)U * 4);
   const size_t b_off = align256((size_t)U * 8);
   const size_t b_status = align256((size_t)(nt + 1) * 8);
   const size_t b_incl = align256((size_t)nt * 16);
   const size_t b_tot = 256;
-  const size_t b_coords = align256((size_t)(g.nx + g.ny + g.nz + 1) * 8);
+  const size_t b_coords = align256((size_t)(ncoords + 1) * 8);
   const size_t need = b_cnt + 2 * b_off + b_status + b_incl + b_tot + b_coords;
   int rc = grow(c, &c->ws, &c->ws_bytes, need, "workspace allocation");
   if (rc) return rc;
@@ -252,6 +282,27 @@ bool same_call(const Ctx* c, const msh_field* f, const msh_params* p) {
          memcmp(&c->params, p, sizeof(*p)) == 0;
 }
 
+bool same_batch(const Ctx* c, int32_t n, const msh_field* f, const msh_params* p) {
+  const BatchState& b = c->batch;
+  return b.have && (size_t)n == b.fields.size() &&
+         (n == 0 || (memcmp(b.fields.data(), f, sizeof(*f) * (size_t)n) == 0 &&
+                     memcmp(b.params.data(), p, sizeof(*p) * (size_t)n) == 0));
+}
+
+// The host copy of the descriptors may be rewritten only once the previous
+// upload from it has run.
+int wait_upload(Ctx* c) {
+  if (!c->upload_pending) return MSH_OK;
+  hipError_t e = hipEventSynchronize(c->upload);
+  if (e != hipSuccess) return hip_fail(c, e, "descriptor upload");
+  c->upload_pending = false;
+  return MSH_OK;
+}
+
+bool same_plan_types(const Plan& a, const Plan& b) {
+  return a.method == b.method && a.T == b.T && a.M == b.M && a.A == b.A && a.Out == b.Out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -263,8 +314,10 @@ int msh_create(int device, void** out) {
   if (!c) return MSH_ERR_OUT_OF_MEMORY;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&c->pinned, 2 * sizeof(int64_t), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->pinned, 4 * sizeof(int64_t), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->upload, hipEventDisableTiming);
   if (e != hipSuccess) {
+    if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
     return e == hipErrorOutOfMemory ? MSH_ERR_OUT_OF_MEMORY : MSH_ERR_DEVICE;
@@ -283,6 +336,9 @@ int msh_destroy(void* ctx) {
   if (c->ws) (void)hipFree(c->ws);
   if (c->fstage) (void)hipFree(c->fstage);
   if (c->ostage) (void)hipFree(c->ostage);
+  if (c->bws) (void)hipFree(c->bws);
+  if (c->hbatch) (void)hipHostFree(c->hbatch);
+  if (c->upload) (void)hipEventDestroy(c->upload);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
@@ -326,13 +382,14 @@ int msh_count(void* ctx, const msh_field* f, const msh_params* p, int64_t* nvert
   Ctx* c = (Ctx*)ctx;
   if (!c) return MSH_ERR_ARGUMENT;
   c->have = false;
+  c->batch.have = false;  // the batch state shares the workspace
   Plan pl;
   int rc = make_plan(c, f, p, &pl);
   if (rc) return rc;
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
   Geom g = make_geom(f);
   Workspace w{};
-  rc = setup_workspace(c, g, &w);
+  rc = setup_workspace(c, g.units, (int64_t)g.nx + g.ny + g.nz, &w);
   if (rc) return rc;
   int64_t nv = 0, nfc = 0;
   const void* dfield = f->data;
@@ -347,23 +404,24 @@ int msh_count(void* ctx, const msh_field* f, const msh_params* p, int64_t* nvert
       if (e != hipSuccess) return hip_fail(c, e, "field upload");
       dfield = c->fstage;
     }
+    const VolArgs a = make_vol(pl, g, dfield, w);
     hipError_t e = launch_coords(pl, g, w.coords, c->stream);
-    if (e == hipSuccess) e = launch_count(pl, g, dfield, w.cnt, c->stream);
+    if (e == hipSuccess) e = launch_count(pl, a, c->stream);
     if (e == hipSuccess) e = launch_scan(g, w, c->stream);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(c->pinned, w.totals, 2 * sizeof(int64_t), hipMemcpyDeviceToHost,
+      e = hipMemcpyAsync(c->pinned, w.totals, 4 * sizeof(int64_t), hipMemcpyDeviceToHost,
                          c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "count/scan");
-    nv = c->pinned[0];
-    nfc = c->pinned[1];
+    nv = c->pinned[1];
+    nfc = c->pinned[3];
+    c->vol = a;
   }
   c->field = *f;
   c->params = *p;
   c->plan = pl;
   c->geom = g;
   c->w = w;
-  c->dfield = dfield;
   c->nverts = nv;
   c->nfaces = nfc;
   c->have = true;
@@ -396,7 +454,10 @@ int msh_extract(void* ctx, const msh_field* f, const msh_params* p, void* verts,
     dv = c->ostage;
     df = (int64_t*)((char*)c->ostage + align256(vbytes));
   }
-  hipError_t e = launch_emit(c->plan, c->geom, c->dfield, c->w, dv, df, c->stream);
+  VolArgs a = c->vol;
+  a.verts = dv;
+  a.faces = df;
+  hipError_t e = launch_emit(c->plan, a, c->stream);
   if (e == hipErrorNotSupported)
     return fail(c, MSH_ERR_UNSUPPORTED, "this method is not available on the GPU path yet");
   if (e != hipSuccess) return hip_fail(c, e, "emit");
@@ -404,6 +465,243 @@ int msh_extract(void* ctx, const msh_field* f, const msh_params* p, void* verts,
     if (vbytes) e = hipMemcpyAsync(verts, dv, vbytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && fbytes)
       e = hipMemcpyAsync(faces, df, fbytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "output download");
+  }
+  return MSH_OK;
+}
+
+int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
+                    int64_t* nverts, int64_t* nfaces) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  BatchState& b = c->batch;
+  b.have = false;
+  c->have = false;  // the single-volume state shares the workspace
+  if (n < 0) return fail(c, MSH_ERR_ARGUMENT, "negative batch size");
+  if (n > 0 && (!fields || !params)) return fail(c, MSH_ERR_ARGUMENT, "null fields or params");
+  int rc = wait_upload(c);
+  if (rc) return rc;
+  b.plans.assign((size_t)n, Plan{});
+  std::vector<Geom> geoms((size_t)n);
+  for (int v = 0; v < n; ++v) {
+    rc = make_plan(c, &fields[v], &params[v], &b.plans[v]);
+    if (rc) {
+      c->err = "volume " + std::to_string(v) + ": " + c->err;
+      return rc;
+    }
+    geoms[v] = make_geom(&fields[v], true);
+    if (geoms[v].units > 0 && !fields[v].data)
+      return fail(c, MSH_ERR_ARGUMENT, "volume " + std::to_string(v) + ": null field data");
+  }
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
+
+  // units, coordinates and host staging, in volume order
+  int64_t units = 0, ncoords = 0;
+  size_t hbytes = 0;
+  std::vector<size_t> hoff((size_t)n, 0);
+  for (int v = 0; v < n; ++v) {
+    units += geoms[v].units;
+    ncoords += (int64_t)geoms[v].nx + geoms[v].ny + geoms[v].nz;
+    if (fields[v].memory == MSH_HOST && geoms[v].units > 0) {
+      hoff[v] = hbytes;
+      hbytes += align256((size_t)fields[v].nx * fields[v].ny * fields[v].nz *
+                         (fields[v].dtype == K_F32 ? 4 : 8));
+    }
+  }
+  Workspace w{};
+  rc = setup_workspace(c, units, ncoords, &w);
+  if (rc) return rc;
+  if (hbytes) {
+    rc = grow(c, &c->fstage, &c->fstage_bytes, hbytes, "field staging allocation");
+    if (rc) return rc;
+  }
+
+  // descriptors; count launches group volumes by (T, method), emit launches
+  // by the full instantiation (types + variant)
+  b.vols.assign((size_t)n, VolArgs{});
+  std::vector<int> emit_var((size_t)n, 0);
+  int64_t ubase = 0, cbase = 0;
+  for (int v = 0; v < n; ++v) {
+    const void* df = fields[v].data;
+    if (fields[v].memory == MSH_HOST && geoms[v].units > 0) df = (char*)c->fstage + hoff[v];
+    VolArgs a = make_vol(b.plans[v], geoms[v], df, w);
+    a.coords = w.coords + cbase;
+    a.ubase = ubase;
+    a.sz = nullptr;  // set below, once the sizes array is placed
+    emit_var[v] = emit_variant(b.plans[v], &a);
+    if (geoms[v].units == 0) {
+      a.cntiles = 0;
+      a.ntiles = 0;
+    }
+    b.vols[v] = a;
+    ubase += geoms[v].units;
+    cbase += (int64_t)geoms[v].nx + geoms[v].ny + geoms[v].nz;
+  }
+  b.counts.clear();
+  b.emits.clear();
+  std::vector<char> done((size_t)n, 0);
+  int64_t cblocks = 0;
+  for (int v = 0; v < n; ++v) {
+    if (done[v]) continue;
+    Launch L;
+    L.plan = b.plans[v];
+    L.gbase = cblocks;
+    for (int u = v; u < n; ++u) {
+      if (done[u] || b.plans[u].T != L.plan.T || b.plans[u].method != L.plan.method) continue;
+      done[u] = 1;
+      b.vols[u].cblk0 = cblocks;
+      cblocks += b.vols[u].cntiles;
+    }
+    L.nblocks = cblocks - L.gbase;
+    if (L.nblocks) b.counts.push_back(L);
+  }
+  done.assign((size_t)n, 0);
+  int64_t eblocks = 0;
+  for (int v = 0; v < n; ++v) {
+    if (done[v]) continue;
+    Launch L;
+    L.plan = b.plans[v];
+    L.variant = emit_var[v];
+    L.gbase = eblocks;
+    for (int u = v; u < n; ++u) {
+      if (done[u] || emit_var[u] != L.variant || !same_plan_types(b.plans[u], L.plan)) continue;
+      done[u] = 1;
+      b.vols[u].eblk0 = eblocks;
+      eblocks += b.vols[u].ntiles;
+    }
+    L.nblocks = eblocks - L.gbase;
+    if (L.nblocks) b.emits.push_back(L);
+  }
+  if (cblocks > INT32_MAX || eblocks > INT32_MAX)
+    return fail(c, MSH_ERR_ARGUMENT, "batch too large for one launch");
+
+  // device side: descriptors | cmap | emap | sizes
+  const size_t b_vols = align256(sizeof(VolArgs) * (size_t)(n > 0 ? n : 1));
+  const size_t b_cmap = align256(4 * (size_t)(cblocks + 1));
+  const size_t b_emap = align256(4 * (size_t)(eblocks + 1));
+  const size_t b_sizes = align256(32 * (size_t)(n > 0 ? n : 1));
+  rc = grow(c, &c->bws, &c->bws_bytes, b_vols + b_cmap + b_emap + b_sizes, "batch workspace");
+  if (rc) return rc;
+  VolArgs* dvols = (VolArgs*)c->bws;
+  int32_t* cmap = (int32_t*)((char*)c->bws + b_vols);
+  int32_t* emap = (int32_t*)((char*)cmap + b_cmap);
+  int64_t* dsizes = (int64_t*)((char*)emap + b_emap);
+  for (int v = 0; v < n; ++v) b.vols[v].sz = dsizes + 4 * v;
+  // pinned host side: descriptors | sizes
+  if (c->hbatch_bytes < b_vols + b_sizes) {
+    if (c->hbatch) (void)hipHostFree(c->hbatch);
+    c->hbatch = nullptr;
+    c->hbatch_bytes = 0;
+    hipError_t e = hipHostMalloc(&c->hbatch, b_vols + b_sizes, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(c, e, "pinned batch buffer");
+    c->hbatch_bytes = b_vols + b_sizes;
+  }
+  int64_t* hsizes = (int64_t*)((char*)c->hbatch + b_vols);
+  if (n > 0) memcpy(c->hbatch, b.vols.data(), sizeof(VolArgs) * (size_t)n);
+
+  hipError_t e = hipSuccess;
+  for (int v = 0; v < n && e == hipSuccess; ++v)
+    if (fields[v].memory == MSH_HOST && geoms[v].units > 0)
+      e = hipMemcpyAsync((char*)c->fstage + hoff[v], fields[v].data,
+                         (size_t)fields[v].nx * fields[v].ny * fields[v].nz *
+                             (fields[v].dtype == K_F32 ? 4 : 8),
+                         hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && n > 0)
+    e = hipMemcpyAsync(dvols, c->hbatch, sizeof(VolArgs) * (size_t)n, hipMemcpyHostToDevice,
+                       c->stream);
+  if (e == hipSuccess) e = launch_fill_maps(dvols, n, cmap, emap, c->stream);
+  if (e == hipSuccess) e = launch_coords_batch(dvols, n, c->stream);
+  for (size_t k = 0; k < b.counts.size() && e == hipSuccess; ++k)
+    e = launch_count_batch(b.counts[k].plan, dvols, cmap, b.counts[k].gbase,
+                           b.counts[k].nblocks, c->stream);
+  Geom gall{};
+  gall.units = units;
+  if (e == hipSuccess && units > 0) e = launch_scan(gall, w, c->stream);
+  if (e == hipSuccess && units > 0) e = launch_sizes_batch(dvols, n, w, units, dsizes, c->stream);
+  if (e == hipSuccess && units > 0)
+    e = hipMemcpyAsync(hsizes, dsizes, 32 * (size_t)n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "batched count/scan");
+
+  b.sizes.assign(4 * (size_t)n, 0);
+  if (units > 0) memcpy(b.sizes.data(), hsizes, 32 * (size_t)n);
+  for (int v = 0; v < n; ++v) {
+    if (nverts) nverts[v] = b.sizes[4 * v + 1];
+    if (nfaces) nfaces[v] = b.sizes[4 * v + 3];
+  }
+  b.fields.assign(fields, fields + n);
+  b.params.assign(params, params + n);
+  b.units = units;
+  b.ncoords = ncoords;
+  b.cblocks = cblocks;
+  b.eblocks = eblocks;
+  b.have = true;
+  return MSH_OK;
+}
+
+int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
+                      void* const* verts, int64_t* const* faces, int32_t out_memory) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  BatchState& b = c->batch;
+  if (n < 0 || (n > 0 && (!fields || !params)))
+    return fail(c, MSH_ERR_ARGUMENT, "null fields or params");
+  if (!same_batch(c, n, fields, params))
+    return fail(c, MSH_ERR_STATE,
+                "msh_batch_extract must follow msh_batch_count with the same arguments");
+  if (out_memory != MSH_HOST && out_memory != MSH_DEVICE)
+    return fail(c, MSH_ERR_ARGUMENT, "out_memory must be MSH_HOST or MSH_DEVICE");
+  if (n == 0 || b.eblocks == 0) return MSH_OK;
+  if (!verts || !faces) return fail(c, MSH_ERR_ARGUMENT, "null output pointer arrays");
+  for (int v = 0; v < n; ++v)
+    if ((b.sizes[4 * v + 1] > 0 && !verts[v]) || (b.sizes[4 * v + 3] > 0 && !faces[v]))
+      return fail(c, MSH_ERR_ARGUMENT, "volume " + std::to_string(v) + ": null output buffer");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
+  int rc = wait_upload(c);
+  if (rc) return rc;
+
+  std::vector<size_t> vb((size_t)n), fb((size_t)n), off((size_t)n);
+  size_t stage = 0;
+  for (int v = 0; v < n; ++v) {
+    const size_t osz = b.plans[v].Out == K_F32 ? 4 : 8;
+    vb[v] = (size_t)b.sizes[4 * v + 1] * 3 * osz;
+    fb[v] = (size_t)b.sizes[4 * v + 3] * 3 * 8;
+    off[v] = stage;
+    stage += align256(vb[v]) + align256(fb[v]);
+  }
+  if (out_memory == MSH_HOST) {
+    rc = grow(c, &c->ostage, &c->ostage_bytes, stage + 256, "output staging allocation");
+    if (rc) return rc;
+  }
+  for (int v = 0; v < n; ++v) {
+    if (out_memory == MSH_HOST) {
+      b.vols[v].verts = (char*)c->ostage + off[v];
+      b.vols[v].faces = (int64_t*)((char*)c->ostage + off[v] + align256(vb[v]));
+    } else {
+      b.vols[v].verts = verts[v];
+      b.vols[v].faces = faces[v];
+    }
+  }
+  VolArgs* dvols = (VolArgs*)c->bws;
+  const size_t b_vols = align256(sizeof(VolArgs) * (size_t)n);
+  const size_t b_cmap = align256(4 * (size_t)(b.cblocks + 1));
+  int32_t* emap = (int32_t*)((char*)c->bws + b_vols + b_cmap);
+  memcpy(c->hbatch, b.vols.data(), sizeof(VolArgs) * (size_t)n);
+  hipError_t e = hipMemcpyAsync(dvols, c->hbatch, sizeof(VolArgs) * (size_t)n,
+                                hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->upload, c->stream);
+  if (e == hipSuccess) c->upload_pending = true;
+  for (size_t k = 0; k < b.emits.size() && e == hipSuccess; ++k)
+    e = launch_emit_batch(b.emits[k].plan, b.emits[k].variant, dvols, emap, b.emits[k].gbase,
+                          b.emits[k].nblocks, c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "batched emit");
+  if (out_memory == MSH_HOST) {
+    for (int v = 0; v < n && e == hipSuccess; ++v) {
+      if (vb[v]) e = hipMemcpyAsync(verts[v], b.vols[v].verts, vb[v], hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess && fb[v])
+        e = hipMemcpyAsync(faces[v], b.vols[v].faces, fb[v], hipMemcpyDeviceToHost, c->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "output download");
   }

@@ -71,19 +71,6 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 }
 
 
-struct EmitArgs {
-  const void* f;
-  Geom g;
-  double thr, iso;         // threshold in T, iso in M
-  double eps, ome;         // MT: eps and one(T)-eps, in A
-  int cxmask;              // bit k: axis k coordinates are Float32
-  const double* coords;    // xp | yp | zp
-  const int64_t* voff;
-  const int64_t* foff;
-  const int64_t* totals;   // (nverts, nfaces): writes beyond are dropped
-  int nti, ntj;            // tiles along x and y
-  int64_t ntiles;
-};
 
 // Inclusive scan over the 64 lanes with DPP: Kogge-Stone within each row of
 // 16 lanes, then row broadcasts of lanes 15 and 31.
@@ -172,9 +159,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uin
   return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
 }
 
-hipError_t launch_count4(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
-                         hipStream_t st);
-hipError_t launch_emit_mc4(const Plan& p, const EmitArgs& a, void* verts, int64_t* faces,
-                           hipStream_t st);
+// lean.hip: the count kernel of every method and the Marching Cubes emit kernel.
+void count4_tiling(const Plan& p, VolArgs* a);
+hipError_t launch_count4(const Plan& p, const VolArgs& a, hipStream_t st);
+hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
+                               int64_t gbase, int64_t nblocks, hipStream_t st);
+// -1: volume too large for k_emit_mc4; else bit 0: dword DMA (not 16-byte),
+// bit 1: 16-cell z chunks for Float32.  Fills the emit tiling.
+int mc4_variant(const Plan& p, VolArgs* a);
+hipError_t launch_emit_mc4(const Plan& p, int variant, const VolArgs& a, hipStream_t st);
+hipError_t launch_emit_mc4_batch(const Plan& p, int variant, const VolArgs* vols,
+                                 const int32_t* emap, int64_t gbase, int64_t nblocks,
+                                 hipStream_t st);
 
 }  // namespace msh

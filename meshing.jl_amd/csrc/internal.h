@@ -43,8 +43,37 @@ struct Workspace {
   uint64_t* status;   // per scan tile: look-back flag + aggregates
   int64_t* incl;      // per scan tile: inclusive prefixes (v, f)
   uint32_t* counter;  // scan tile ticket
-  int64_t* totals;    // (nverts, nfaces)
+  int64_t* totals;    // (0, nverts, 0, nfaces): the sizes of a single volume
   double* coords;     // xp[nx], yp[ny], zp[nz] as double
+};
+
+// Everything the kernels need to know about one volume.  Single-volume
+// launches take it by value; batched launches read one entry per volume
+// from an array in device memory.
+struct VolArgs {
+  const void* f;           // field, x fastest
+  Geom g;
+  double thr, iso;         // classification threshold in T, iso in M
+  double eps, ome;         // MT: eps and one(T)-eps, in A
+  int32_t cxmask;          // bit k: axis k coordinates are Float32
+  int32_t method;
+  double ca[3], cb[3];     // LinRange ends per axis
+  double* coords;          // xp | yp | zp of this volume
+  uint32_t* cnt;           // unit counts, indexed from ubase
+  const int64_t* voff;     // exclusive unit prefixes, indexed from ubase
+  const int64_t* foff;
+  const int64_t* sz;       // (first vertex, nverts, first face, nfaces) of this
+                           // volume in voff/foff numbering; writes past the
+                           // counts are dropped
+  void* verts;             // this volume's outputs
+  int64_t* faces;
+  int64_t ubase;           // first unit of this volume
+  int32_t cnbx, cnby;      // count tiles along x and y
+  int64_t cntiles;         // count tiles (x * y * z segments)
+  int64_t cblk0;           // first count block of this volume in a batched launch
+  int32_t nti, ntj;        // emit tiles along x and y
+  int64_t ntiles;          // emit tiles
+  int64_t eblk0;           // first emit block of this volume in a batched launch
 };
 
 constexpr int kScanItems = 8;
@@ -52,12 +81,33 @@ constexpr int kScanTile = 256 * kScanItems;
 
 inline int64_t scan_tiles(int64_t units) { return (units + kScanTile - 1) / kScanTile; }
 
+// VolArgs of a single extraction (sz = w.totals, ubase = 0), with the
+// count and emit tilings filled in.
+VolArgs make_vol(const Plan& p, const Geom& g, const void* field, const Workspace& w);
+// Emit kernel variant of a volume (volumes of one batched emit launch share
+// it) and its tiling; fills nti, ntj, ntiles.
+int emit_variant(const Plan& p, VolArgs* a);
+void count_tiling(const Plan& p, VolArgs* a);
+
 hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream_t st);
-hipError_t launch_count(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
-                        hipStream_t st);
+hipError_t launch_count(const Plan& p, const VolArgs& a, hipStream_t st);
 hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st);
-hipError_t launch_emit(const Plan& p, const Geom& g, const void* field, const Workspace& w,
-                       void* verts, int64_t* faces, hipStream_t st);
+hipError_t launch_emit(const Plan& p, const VolArgs& a, hipStream_t st);
+
+// Batched launches over n volumes whose VolArgs are in device memory.
+// Volumes of one count/emit launch share the kernel instantiation (same
+// Plan types and emit variant); gbase is the launch's first block in the
+// block->volume maps.
+hipError_t launch_coords_batch(const VolArgs* vols, int n, hipStream_t st);
+hipError_t launch_fill_maps(const VolArgs* vols, int n, int32_t* cmap, int32_t* emap,
+                            hipStream_t st);
+hipError_t launch_count_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
+                              int64_t gbase, int64_t nblocks, hipStream_t st);
+hipError_t launch_sizes_batch(const VolArgs* vols, int n, const Workspace& w, int64_t units,
+                              int64_t* sizes, hipStream_t st);
+hipError_t launch_emit_batch(const Plan& p, int variant, const VolArgs* vols,
+                             const int32_t* emap, int64_t gbase, int64_t nblocks,
+                             hipStream_t st);
 hipError_t launch_generate(int kind, int dtype, int64_t nx, int64_t ny, int64_t nz,
                            uint64_t seed, double p0, double p1, void* out, hipStream_t st);
 

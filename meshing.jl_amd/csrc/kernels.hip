@@ -32,26 +32,67 @@ namespace {
 // ---------------------------------------------------------------------------
 // LinRange coordinates: xp[i] = CX((1-t)*a + t*b), t = (i-1)/max(1, n-1)
 // (Julia Base lerpi; used at src/marching_cubes.jl:36-38).
+__device__ __forceinline__ double lerpi(int j, int n, double a, double b, bool f32) {
+  const int d = n - 1 > 1 ? n - 1 : 1;
+  const double t = (double)j / (double)d;
+  const double v = (1.0 - t) * a + t * b;
+  return f32 ? (double)(float)v : v;
+}
+
 __global__ void k_coords(double* __restrict__ out, int nx, int ny, int nz, double ax, double bx,
                          double ay, double by, double az, double bz, int cxmask) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int axis, n, j;
-  if (i < nx) {
-    axis = 0; n = nx; j = i;
-  } else if (i < nx + ny) {
-    axis = 1; n = ny; j = i - nx;
-  } else if (i < nx + ny + nz) {
+  if (i >= nx + ny + nz) return;
+  const double ca[3] = {ax, ay, az}, cb[3] = {bx, by, bz};
+  int axis = 0, n = nx, j = i;
+  if (i >= nx + ny) {
     axis = 2; n = nz; j = i - nx - ny;
-  } else {
-    return;
+  } else if (i >= nx) {
+    axis = 1; n = ny; j = i - nx;
   }
-  const double a = axis == 0 ? ax : axis == 1 ? ay : az;
-  const double b = axis == 0 ? bx : axis == 1 ? by : bz;
-  const int d = n - 1 > 1 ? n - 1 : 1;
-  const double t = (double)j / (double)d;
-  double v = (1.0 - t) * a + t * b;
-  if ((cxmask >> axis) & 1) v = (double)(float)v;
-  out[i] = v;
+  out[i] = lerpi(j, n, ca[axis], cb[axis], (cxmask >> axis) & 1);
+}
+
+// Batched: one block per volume.
+__global__ void k_coords_b(const VolArgs* __restrict__ vols) {
+  const VolArgs& a = vols[blockIdx.x];
+  const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+  for (int i = threadIdx.x; i < nx + ny + nz; i += blockDim.x) {
+    int axis = 0, n = nx, j = i;
+    if (i >= nx + ny) {
+      axis = 2; n = nz; j = i - nx - ny;
+    } else if (i >= nx) {
+      axis = 1; n = ny; j = i - nx;
+    }
+    a.coords[i] = lerpi(j, n, a.ca[axis], a.cb[axis], (a.cxmask >> axis) & 1);
+  }
+}
+
+// Block -> volume maps of a batch: one block per volume fills its ranges.
+__global__ void k_fill_maps(const VolArgs* __restrict__ vols, int32_t* __restrict__ cmap,
+                            int32_t* __restrict__ emap) {
+  const int v = blockIdx.x;
+  const VolArgs& a = vols[v];
+  for (int64_t i = threadIdx.x; i < a.cntiles; i += blockDim.x) cmap[a.cblk0 + i] = v;
+  for (int64_t i = threadIdx.x; i < a.ntiles; i += blockDim.x) emap[a.eblk0 + i] = v;
+}
+
+// Per-volume sizes of a batch from the scan over all volumes' units (volume
+// v's units start at ubase and volumes are stored in unit order):
+// (first vertex, nverts, first face, nfaces).
+__global__ void k_sizes(const VolArgs* __restrict__ vols, int n, const int64_t* __restrict__ voff,
+                        const int64_t* __restrict__ foff, const int64_t* __restrict__ totals,
+                        int64_t units, int64_t* __restrict__ sizes) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int64_t ub = vols[v].ubase;
+  const int64_t ue = v + 1 < n ? vols[v + 1].ubase : units;
+  const int64_t vb = ub < units ? voff[ub] : totals[1], fb = ub < units ? foff[ub] : totals[3];
+  const int64_t ve = ue < units ? voff[ue] : totals[1], fe = ue < units ? foff[ue] : totals[3];
+  sizes[4 * v + 0] = vb;
+  sizes[4 * v + 1] = ve - vb;
+  sizes[4 * v + 2] = fb;
+  sizes[4 * v + 3] = fe - fb;
 }
 
 // ---------------------------------------------------------------------------
@@ -168,8 +209,10 @@ __global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ cn
       s_excl[0] = ev;
       s_excl[1] = ef;
       if (tile == ntiles - 1) {
-        totals[0] = ev + agg_v;
-        totals[1] = ef + agg_f;
+        totals[0] = 0;
+        totals[1] = ev + agg_v;
+        totals[2] = 0;
+        totals[3] = ef + agg_f;
       }
     }
   }
@@ -233,17 +276,17 @@ struct EmitSmem {
 };
 
 template <int METHOD, typename T, typename M, typename A, typename Out, int TX, int TY>
-__global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, Out* __restrict__ verts,
-                                                 int64_t* __restrict__ faces) {
+__device__ __forceinline__ void emit_tile(const VolArgs& a, int64_t t) {
   using C = EmitCfg<METHOD, T, TX, TY>;
   constexpr int H = C::H, RX = C::RX, NR = C::NR, SX = C::SX, SY = C::SY, NP = C::NP,
                 PS = C::PS, CAP = C::CAP;
   __shared__ EmitSmem<METHOD, T, TX, TY> sm;
   const T* __restrict__ f = (const T*)a.f;
+  Out* __restrict__ verts = (Out*)a.verts;
+  int64_t* __restrict__ faces = a.faces;
   const Geom g = a.g;
   const int lane = lane_id(), wv = wave_id();
 
-  const int64_t t = xcd_tile(blockIdx.x, a.ntiles);
   const int tj = (int)(t % a.ntj);
   const int64_t rest = t / a.ntj;
   const int ti = (int)(rest % a.nti);
@@ -256,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, Out* __restrict__ v
   const double* xp = a.coords;
   const double* yp = a.coords + g.nx;
   const double* zp = a.coords + g.nx + g.ny;
-  const int64_t totv = a.totals[0], totf = a.totals[1];
+  const int64_t vbase = a.sz[0], totv = a.sz[1], fbase = a.sz[2], totf = a.sz[3];
   const int64_t pstride = (int64_t)g.nx * g.ny;
 
   if constexpr (METHOD == MC) {
@@ -270,9 +313,9 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, Out* __restrict__ v
     const int i = x0 + r % RX, j = y0 + r / RX;
     int64_t bv = 0, bf = 0;
     if (i >= 0 && j >= 0 && i < g.nx - 1 && j < g.ny - 1) {
-      const int64_t u = ((int64_t)i * (g.ny - 1) + j) * g.nzseg + s;
-      bv = a.voff[u];
-      bf = a.foff[u];
+      const int64_t u = a.ubase + ((int64_t)i * (g.ny - 1) + j) * g.nzseg + s;
+      bv = a.voff[u] - vbase;
+      bf = a.foff[u] - fbase;
     }
     sm.rbv[0][r] = bv;
     sm.rbf[0][r] = bf;
@@ -623,6 +666,20 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, Out* __restrict__ v
   }
 }
 
+template <int METHOD, typename T, typename M, typename A, typename Out, int TX, int TY>
+__global__ __launch_bounds__(kBlock) void k_emit(VolArgs a) {
+  emit_tile<METHOD, T, M, A, Out, TX, TY>(a, xcd_tile(blockIdx.x, a.ntiles));
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out, int TX, int TY>
+__global__ __launch_bounds__(kBlock) void k_emit_b(const VolArgs* __restrict__ vols,
+                                                   const int32_t* __restrict__ emap,
+                                                   int64_t gbase) {
+  const int64_t b = gbase + blockIdx.x;
+  const VolArgs a = vols[emap[b]];
+  emit_tile<METHOD, T, M, A, Out, TX, TY>(a, b - a.eblk0);
+}
+
 // ---------------------------------------------------------------------------
 // Synthetic fields (no reference counterpart): column-major, x fastest.
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
@@ -721,11 +778,6 @@ hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_count(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
-                        hipStream_t st) {
-  return launch_count4(p, g, field, cnt, st);
-}
-
 hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st) {
   const int64_t nt = scan_tiles(g.units);
   // status[nt] and the ticket counter are contiguous: status | counter
@@ -737,20 +789,9 @@ hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int METHOD, typename T, typename M, typename A, typename Out, int TX, int TY>
-static hipError_t emit_t(const EmitArgs& a, void* verts, int64_t* faces, hipStream_t st) {
-  EmitArgs b = a;
-  b.nti = (a.g.nx - 1 + TX - 1) / TX;
-  b.ntj = (a.g.ny - 1 + TY - 1) / TY;
-  b.ntiles = (int64_t)b.nti * b.ntj * a.g.nzseg;
-  hipLaunchKernelGGL((k_emit<METHOD, T, M, A, Out, TX, TY>), dim3((unsigned)b.ntiles),
-                     dim3(kBlock), 0, st, b, (Out*)verts, faces);
-  return hipGetLastError();
-}
-
-// Brick shape (cells along x, y) of the emit kernel.  MSH_EMIT_SHAPE picks
-// one of the Marching Cubes shapes (0: 31x2, 1: 31x1, 2: 15x2, 3: 15x1) for
-// tuning; the default is per element type.
+// Brick shape (cells along x, y) of the generic emit kernel.  MSH_EMIT_SHAPE
+// picks one of the Marching Cubes shapes (0: 31x2, 1: 31x1, 2: 15x2,
+// 3: 15x1) for tuning; the default is per element type.
 static int emit_shape_choice(int dflt) {
   const char* e = getenv("MSH_EMIT_SHAPE");
   if (!e || !*e) return dflt;
@@ -758,38 +799,34 @@ static int emit_shape_choice(int dflt) {
   return (v >= 0 && v <= 3) ? v : dflt;
 }
 
-template <int METHOD, typename T, typename M, typename A, typename Out>
-static hipError_t emit_shape(const EmitArgs& a, void* verts, int64_t* faces, hipStream_t st) {
-  if constexpr (METHOD == MC) {
-    switch (emit_shape_choice(sizeof(T) == 4 ? 1 : 3)) {
-      case 0: return emit_t<MC, T, M, A, Out, 31, 2>(a, verts, faces, st);
-      case 1: return emit_t<MC, T, M, A, Out, 31, 1>(a, verts, faces, st);
-      case 2: return emit_t<MC, T, M, A, Out, 15, 2>(a, verts, faces, st);
-      default: return emit_t<MC, T, M, A, Out, 15, 1>(a, verts, faces, st);
+static int generic_shape(const Plan& p) {
+  if (p.method == MC) return emit_shape_choice(p.T == K_F32 ? 1 : 3);
+  return p.T == K_F32 ? 1 : 3;
+}
+
+static const int kShapeTX[4] = {31, 31, 15, 15}, kShapeTY[4] = {2, 1, 2, 1};
+
+// Variants: 0..11 = k_emit_mc4 (bit 0 dword DMA, bit 1 16-cell chunks, bits
+// 2-3 axis types: 1 all Float64, 2 all Float32, 0 mixed); 16 + shape = k_emit.
+int emit_variant(const Plan& p, VolArgs* a) {
+  if (p.method == MC) {
+    const char* legacy = getenv("MSH_EMIT_V3");
+    if (!(legacy && *legacy == '1')) {
+      const int v = mc4_variant(p, a);
+      if (v >= 0) return v;
     }
-  } else if constexpr (sizeof(T) == 4) {
-    return emit_t<METHOD, T, M, A, Out, 31, 1>(a, verts, faces, st);
-  } else {
-    return emit_t<METHOD, T, M, A, Out, 15, 1>(a, verts, faces, st);
   }
+  const int sh = generic_shape(p);
+  a->nti = (a->g.nx - 1 + kShapeTX[sh] - 1) / kShapeTX[sh];
+  a->ntj = (a->g.ny - 1 + kShapeTY[sh] - 1) / kShapeTY[sh];
+  a->ntiles = (a->g.nx > 1 && a->g.ny > 1) ? (int64_t)a->nti * a->ntj * a->g.nzseg : 0;
+  return 16 + sh;
 }
 
-template <int METHOD>
-static hipError_t emit_types(const Plan& p, const EmitArgs& a, void* verts, int64_t* faces,
-                             hipStream_t st) {
-  if (p.T == K_F32 && p.M == K_F32 && p.A == K_F32 && p.Out == K_F32)
-    return emit_shape<METHOD, float, float, float, float>(a, verts, faces, st);
-  if (p.T == K_F32 && p.M == K_F32 && p.A == K_F32)
-    return emit_shape<METHOD, float, float, float, double>(a, verts, faces, st);
-  if (p.T == K_F32 && p.M == K_F32)
-    return emit_shape<METHOD, float, float, double, double>(a, verts, faces, st);
-  if (p.T == K_F32) return emit_shape<METHOD, float, double, double, double>(a, verts, faces, st);
-  return emit_shape<METHOD, double, double, double, double>(a, verts, faces, st);
-}
+void count_tiling(const Plan& p, VolArgs* a) { count4_tiling(p, a); }
 
-hipError_t launch_emit(const Plan& p, const Geom& g, const void* field, const Workspace& w,
-                       void* verts, int64_t* faces, hipStream_t st) {
-  EmitArgs a{};
+VolArgs make_vol(const Plan& p, const Geom& g, const void* field, const Workspace& w) {
+  VolArgs a{};
   a.f = field;
   a.g = g;
   a.thr = p.thr;
@@ -797,26 +834,122 @@ hipError_t launch_emit(const Plan& p, const Geom& g, const void* field, const Wo
   a.eps = p.epsA;
   a.ome = p.omeA;
   a.cxmask = (p.cx[0] == K_F32) | ((p.cx[1] == K_F32) << 1) | ((p.cx[2] == K_F32) << 2);
+  a.method = p.method;
+  for (int k = 0; k < 3; ++k) {
+    a.ca[k] = p.a[k];
+    a.cb[k] = p.b[k];
+  }
   a.coords = w.coords;
+  a.cnt = w.cnt;
   a.voff = w.voff;
   a.foff = w.foff;
-  a.totals = w.totals;
-  if (p.method == MC) {
-    const char* legacy = getenv("MSH_EMIT_V3");
-    if (!(legacy && *legacy == '1')) {
-      const hipError_t e = launch_emit_mc4(p, a, verts, faces, st);
-      if (e != hipErrorNotSupported) return e;
+  a.sz = w.totals;
+  a.ubase = 0;
+  count_tiling(p, &a);
+  return a;
+}
+
+hipError_t launch_count(const Plan& p, const VolArgs& a, hipStream_t st) {
+  return launch_count4(p, a, st);
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out, int TX, int TY>
+static hipError_t emit_t(const VolArgs& a, const VolArgs* vols, const int32_t* emap,
+                         int64_t gbase, int64_t nblocks, hipStream_t st) {
+  if (vols)
+    hipLaunchKernelGGL((k_emit_b<METHOD, T, M, A, Out, TX, TY>), dim3((unsigned)nblocks),
+                       dim3(kBlock), 0, st, vols, emap, gbase);
+  else
+    hipLaunchKernelGGL((k_emit<METHOD, T, M, A, Out, TX, TY>), dim3((unsigned)a.ntiles),
+                       dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out>
+static hipError_t emit_shape(int sh, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
+                             int64_t gbase, int64_t nblocks, hipStream_t st) {
+  if constexpr (METHOD == MC) {
+    switch (sh) {
+      case 0: return emit_t<MC, T, M, A, Out, 31, 2>(a, vols, emap, gbase, nblocks, st);
+      case 1: return emit_t<MC, T, M, A, Out, 31, 1>(a, vols, emap, gbase, nblocks, st);
+      case 2: return emit_t<MC, T, M, A, Out, 15, 2>(a, vols, emap, gbase, nblocks, st);
+      default: return emit_t<MC, T, M, A, Out, 15, 1>(a, vols, emap, gbase, nblocks, st);
     }
-    Plan q = p;
-    q.A = q.M;  // no eps in Marching Cubes
-    return emit_types<MC>(q, a, verts, faces, st);
+  } else if constexpr (sizeof(T) == 4) {
+    return emit_t<METHOD, T, M, A, Out, 31, 1>(a, vols, emap, gbase, nblocks, st);
+  } else {
+    return emit_t<METHOD, T, M, A, Out, 15, 1>(a, vols, emap, gbase, nblocks, st);
   }
-  if (p.method == SN) {
-    Plan q = p;
-    q.A = q.M;
-    return emit_types<SN>(q, a, verts, faces, st);
+}
+
+template <int METHOD>
+static hipError_t emit_types(const Plan& p, int sh, const VolArgs& a, const VolArgs* vols,
+                             const int32_t* emap, int64_t gbase, int64_t nblocks,
+                             hipStream_t st) {
+#define MSH_EMIT_ARGS sh, a, vols, emap, gbase, nblocks, st
+  if (p.T == K_F32 && p.M == K_F32 && p.A == K_F32 && p.Out == K_F32)
+    return emit_shape<METHOD, float, float, float, float>(MSH_EMIT_ARGS);
+  if (p.T == K_F32 && p.M == K_F32 && p.A == K_F32)
+    return emit_shape<METHOD, float, float, float, double>(MSH_EMIT_ARGS);
+  if (p.T == K_F32 && p.M == K_F32)
+    return emit_shape<METHOD, float, float, double, double>(MSH_EMIT_ARGS);
+  if (p.T == K_F32) return emit_shape<METHOD, float, double, double, double>(MSH_EMIT_ARGS);
+  return emit_shape<METHOD, double, double, double, double>(MSH_EMIT_ARGS);
+#undef MSH_EMIT_ARGS
+}
+
+static hipError_t emit_any(const Plan& p, int variant, const VolArgs& a, const VolArgs* vols,
+                           const int32_t* emap, int64_t gbase, int64_t nblocks, hipStream_t st) {
+  if (variant < 16) {
+    if (vols) return launch_emit_mc4_batch(p, variant, vols, emap, gbase, nblocks, st);
+    return launch_emit_mc4(p, variant, a, st);
   }
-  return emit_types<MT>(p, a, verts, faces, st);
+  if (!vols && a.ntiles == 0) return hipSuccess;
+  if (vols && nblocks == 0) return hipSuccess;
+  const int sh = variant - 16;
+  Plan q = p;
+  if (p.method != MT) q.A = q.M;  // eps is used by Marching Tetrahedra only
+  if (p.method == MC) return emit_types<MC>(q, sh, a, vols, emap, gbase, nblocks, st);
+  if (p.method == SN) return emit_types<SN>(q, sh, a, vols, emap, gbase, nblocks, st);
+  return emit_types<MT>(q, sh, a, vols, emap, gbase, nblocks, st);
+}
+
+hipError_t launch_emit(const Plan& p, const VolArgs& a0, hipStream_t st) {
+  VolArgs a = a0;
+  const int variant = emit_variant(p, &a);
+  return emit_any(p, variant, a, nullptr, nullptr, 0, 0, st);
+}
+
+hipError_t launch_coords_batch(const VolArgs* vols, int n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_coords_b, dim3((unsigned)n), dim3(256), 0, st, vols);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_maps(const VolArgs* vols, int n, int32_t* cmap, int32_t* emap,
+                            hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_maps, dim3((unsigned)n), dim3(256), 0, st, vols, cmap, emap);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
+                              int64_t gbase, int64_t nblocks, hipStream_t st) {
+  return launch_count4_batch(p, vols, cmap, gbase, nblocks, st);
+}
+
+hipError_t launch_sizes_batch(const VolArgs* vols, int n, const Workspace& w, int64_t units,
+                              int64_t* sizes, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sizes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vols, n,
+                     w.voff, w.foff, w.totals, units, sizes);
+  return hipGetLastError();
+}
+
+hipError_t launch_emit_batch(const Plan& p, int variant, const VolArgs* vols,
+                             const int32_t* emap, int64_t gbase, int64_t nblocks,
+                             hipStream_t st) {
+  return emit_any(p, variant, VolArgs{}, vols, emap, gbase, nblocks, st);
 }
 
 hipError_t launch_generate(int kind, int dtype, int64_t nx, int64_t ny, int64_t nz,

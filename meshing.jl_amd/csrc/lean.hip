@@ -63,10 +63,13 @@ __device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, 
 // even positions, and ORs in the x+1 column's bits from the next lane at
 // odd positions; cell row r's face nibble is then bits [2r, 2r+4).
 template <typename T, int METHOD, int TY, int ZU>
-__global__ __launch_bounds__(kBlock) void k_count4(const T* __restrict__ f, Geom g, T thr,
-                                                   uint32_t* __restrict__ cnt, int nbx, int nby,
-                                                   int64_t nblocks) {
+__device__ __forceinline__ void count4_tile(const VolArgs& a, int64_t t) {
   static_assert(2 * (TY + 1) <= 32, "mask must fit 32 bits");
+  const T* __restrict__ f = (const T*)a.f;
+  const Geom g = a.g;
+  const T thr = (T)a.thr;
+  uint32_t* __restrict__ cnt = a.cnt + a.ubase;
+  const int nbx = a.cnbx, nby = a.cnby;
   constexpr int NL = METHOD == MC ? 256 : 2048;
   __shared__ uint32_t lut[NL];  // vertices | faces << 16, by class << 8 | natural case
   for (int c = threadIdx.x; c < NL; c += kBlock) {
@@ -97,7 +100,6 @@ __global__ __launch_bounds__(kBlock) void k_count4(const T* __restrict__ f, Geom
   __syncthreads();
 
   const int lane = lane_id(), wv = wave_id();
-  const int64_t t = xcd_tile(blockIdx.x, nblocks);
   const int by = (int)(t % nby);
   const int64_t rest = t / nby;
   const int bx = (int)(rest % nbx);
@@ -192,6 +194,22 @@ __global__ __launch_bounds__(kBlock) void k_count4(const T* __restrict__ f, Geom
   }
 }
 
+template <typename T, int METHOD, int TY, int ZU>
+__global__ __launch_bounds__(kBlock) void k_count4(VolArgs a) {
+  count4_tile<T, METHOD, TY, ZU>(a, xcd_tile(blockIdx.x, a.cntiles));
+}
+
+// Batched: block b of the launch is block gbase + b of the batch; the map
+// names its volume, whose tiles start at block cblk0.
+template <typename T, int METHOD, int TY, int ZU>
+__global__ __launch_bounds__(kBlock) void k_count4_b(const VolArgs* __restrict__ vols,
+                                                     const int32_t* __restrict__ cmap,
+                                                     int64_t gbase) {
+  const int64_t b = gbase + blockIdx.x;
+  const VolArgs a = vols[cmap[b]];
+  count4_tile<T, METHOD, TY, ZU>(a, b - a.cblk0);
+}
+
 // ---------------------------------------------------------------------------
 // Emit, Marching Cubes.
 //
@@ -211,9 +229,8 @@ __global__ __launch_bounds__(kBlock) void k_count4(const T* __restrict__ f, Geom
 //    slot o), finds its cell by binary search over the list prefixes, and
 //    writes output rbv[row] + slot -- consecutive lanes write consecutive
 //    outputs of a row.
-template <typename T, typename M, typename Out, int ZC, bool VEC>
-__global__ __launch_bounds__(kBlock) void k_emit_mc4(EmitArgs a, Out* __restrict__ verts,
-                                                     int64_t* __restrict__ faces) {
+template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM>
+__device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
   constexpr int TX = kEmitTX, SX = 64, SY = 2;
   constexpr int R = kWave / ZC;
   constexpr int NP = ZC + 1;
@@ -237,9 +254,10 @@ __global__ __launch_bounds__(kBlock) void k_emit_mc4(EmitArgs a, Out* __restrict
   };
   __shared__ Smem sm;
   const T* __restrict__ f = (const T*)a.f;
+  Out* __restrict__ verts = (Out*)a.verts;
+  int64_t* __restrict__ faces = a.faces;
   const Geom g = a.g;
   const int lane = lane_id(), wv = wave_id();
-  const int64_t t = xcd_tile(blockIdx.x, a.ntiles);
   const int jb = (int)(t % a.ntj);
   const int64_t rest = t / a.ntj;
   const int ib = (int)(rest % a.nti) * TX;
@@ -247,9 +265,12 @@ __global__ __launch_bounds__(kBlock) void k_emit_mc4(EmitArgs a, Out* __restrict
   const int zs = s * g.zseg, ze = min(zs + g.zseg, g.nz - 1);
   const T thr = (T)a.thr;
   const M iso = (M)a.iso;
-  const bool cx32 = a.cxmask & 1, cy32 = (a.cxmask >> 1) & 1, cz32 = (a.cxmask >> 2) & 1;
+  // axis element types: compile-time when CXM >= 0 (all Float64: 0, all
+  // Float32: 7), else read from the arguments
+  const int cxm = CXM >= 0 ? CXM : a.cxmask;
+  const bool cx32 = cxm & 1, cy32 = (cxm >> 1) & 1, cz32 = (cxm >> 2) & 1;
   const double* zp = a.coords + g.nx + g.ny;
-  const int64_t totv = a.totals[0], totf = a.totals[1];
+  const int64_t vbase = a.sz[0], totv = a.sz[1], fbase = a.sz[2], totf = a.sz[3];
   const int64_t nelem = (int64_t)g.nx * g.ny * g.nz;
   const int64_t pstride = (int64_t)g.nx * g.ny;
 
@@ -264,9 +285,9 @@ __global__ __launch_bounds__(kBlock) void k_emit_mc4(EmitArgs a, Out* __restrict
   for (int c = threadIdx.x; c < SX; c += kBlock) {
     int64_t bv = 0, bf = 0;
     if (c < TX && ib + c < g.nx - 1) {
-      const int64_t u = ((int64_t)(ib + c) * (g.ny - 1) + jb) * g.nzseg + s;
-      bv = a.voff[u];
-      bf = a.foff[u];
+      const int64_t u = a.ubase + ((int64_t)(ib + c) * (g.ny - 1) + jb) * g.nzseg + s;
+      bv = a.voff[u] - vbase;
+      bf = a.foff[u] - fbase;
     }
     sm.rbv[0][c] = bv;
     sm.rbf[0][c] = bf;
@@ -482,62 +503,172 @@ __global__ __launch_bounds__(kBlock) void k_emit_mc4(EmitArgs a, Out* __restrict
   }
 }
 
+template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM>
+__global__ __launch_bounds__(kBlock) void k_emit_mc4(VolArgs a) {
+  emit_mc4_tile<T, M, Out, ZC, VEC, CXM>(a, xcd_tile(blockIdx.x, a.ntiles));
+}
+
+template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM>
+__global__ __launch_bounds__(kBlock) void k_emit_mc4_b(const VolArgs* __restrict__ vols,
+                                                       const int32_t* __restrict__ emap,
+                                                       int64_t gbase) {
+  const int64_t b = gbase + blockIdx.x;
+  const VolArgs a = vols[emap[b]];
+  emit_mc4_tile<T, M, Out, ZC, VEC, CXM>(a, b - a.eblk0);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Host side.
+
+template <typename T>
+static constexpr int count_ty() { return sizeof(T) == 4 ? 8 : 6; }
+constexpr int kCountZU = 4;
+
+void count4_tiling(const Plan& p, VolArgs* a) {
+  const int ty = p.T == K_F32 ? count_ty<float>() : count_ty<double>();
+  const int64_t cx = a->g.nx - 1, cy = a->g.ny - 1;
+  if (cx <= 0 || cy <= 0 || a->g.nzseg <= 0) {
+    a->cnbx = a->cnby = 0;
+    a->cntiles = 0;
+    return;
+  }
+  a->cnbx = (int)((cx + kWaves * (kWave - 1) - 1) / (kWaves * (kWave - 1)));
+  a->cnby = (int)((cy + ty - 1) / ty);
+  a->cntiles = (int64_t)a->cnbx * a->cnby * a->g.nzseg;
+}
 
 template <typename T, int METHOD>
-static hipError_t count4_t(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
-                           hipStream_t st) {
-  constexpr int TY = sizeof(T) == 4 ? 8 : 6, ZU = 4;
-  const int nbx = (g.nx - 1 + kWaves * (kWave - 1) - 1) / (kWaves * (kWave - 1));
-  const int nby = (g.ny - 1 + TY - 1) / TY;
-  const int64_t nb = (int64_t)nbx * nby * g.nzseg;
-  hipLaunchKernelGGL((k_count4<T, METHOD, TY, ZU>), dim3((unsigned)nb), dim3(kBlock), 0, st,
-                     (const T*)field, g, (T)p.thr, cnt, nbx, nby, nb);
+static hipError_t count4_t(const VolArgs& a, const VolArgs* vols, const int32_t* cmap,
+                           int64_t gbase, int64_t nblocks, hipStream_t st) {
+  constexpr int TY = count_ty<T>();
+  if (vols)
+    hipLaunchKernelGGL((k_count4_b<T, METHOD, TY, kCountZU>), dim3((unsigned)nblocks),
+                       dim3(kBlock), 0, st, vols, cmap, gbase);
+  else
+    hipLaunchKernelGGL((k_count4<T, METHOD, TY, kCountZU>), dim3((unsigned)a.cntiles),
+                       dim3(kBlock), 0, st, a);
   return hipGetLastError();
 }
 
-hipError_t launch_count4(const Plan& p, const Geom& g, const void* field, uint32_t* cnt,
-                         hipStream_t st) {
+static hipError_t count4_any(const Plan& p, const VolArgs& a, const VolArgs* vols,
+                             const int32_t* cmap, int64_t gbase, int64_t nblocks,
+                             hipStream_t st) {
   if (p.T == K_F32) {
-    if (p.method == MC) return count4_t<float, MC>(p, g, field, cnt, st);
-    if (p.method == MT) return count4_t<float, MT>(p, g, field, cnt, st);
-    return count4_t<float, SN>(p, g, field, cnt, st);
+    if (p.method == MC) return count4_t<float, MC>(a, vols, cmap, gbase, nblocks, st);
+    if (p.method == MT) return count4_t<float, MT>(a, vols, cmap, gbase, nblocks, st);
+    return count4_t<float, SN>(a, vols, cmap, gbase, nblocks, st);
   }
-  if (p.method == MC) return count4_t<double, MC>(p, g, field, cnt, st);
-  if (p.method == MT) return count4_t<double, MT>(p, g, field, cnt, st);
-  return count4_t<double, SN>(p, g, field, cnt, st);
+  if (p.method == MC) return count4_t<double, MC>(a, vols, cmap, gbase, nblocks, st);
+  if (p.method == MT) return count4_t<double, MT>(a, vols, cmap, gbase, nblocks, st);
+  return count4_t<double, SN>(a, vols, cmap, gbase, nblocks, st);
+}
+
+hipError_t launch_count4(const Plan& p, const VolArgs& a, hipStream_t st) {
+  if (a.cntiles == 0) return hipSuccess;
+  return count4_any(p, a, nullptr, nullptr, 0, 0, st);
+}
+
+hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
+                               int64_t gbase, int64_t nblocks, hipStream_t st) {
+  if (nblocks == 0) return hipSuccess;
+  return count4_any(p, VolArgs{}, vols, cmap, gbase, nblocks, st);
+}
+
+// z chunk of the Marching Cubes emit: 32 (Float32) or 16 cells.  Float32
+// also has a 16-cell form with half the LDS brick; MSH_MC4_ZC=16 selects it
+// (for tuning).
+static int mc4_zc(int T) {
+  if (T != K_F32) return 16;
+  static const int zc = [] {
+    const char* e = getenv("MSH_MC4_ZC");
+    return (e && atoi(e) == 16) ? 16 : 32;
+  }();
+  return zc;
+}
+
+int mc4_variant(const Plan& p, VolArgs* a) {
+  const int zc = mc4_zc(p.T);
+  const int64_t esz = p.T == K_F32 ? 4 : 8;
+  // the brick's DMA offsets are 32-bit and span ZC+1 planes
+  if ((int64_t)(zc + 1) * a->g.nx * a->g.ny * esz >= (1ll << 31)) return -1;
+  a->nti = (a->g.nx - 1 + kEmitTX - 1) / kEmitTX;
+  a->ntj = a->g.ny - 1;
+  a->ntiles = (int64_t)a->nti * a->ntj * a->g.nzseg;
+  // 16-byte DMA needs 16-byte aligned sample rows
+  const bool vec = ((uintptr_t)a->f % 16 == 0) && (((int64_t)a->g.nx * esz) % 16 == 0);
+  const int cxclass = a->cxmask == 0 ? 1 : a->cxmask == 7 ? 2 : 0;
+  return (vec ? 0 : 1) | (zc == 16 && p.T == K_F32 ? 2 : 0) | (cxclass << 2);
+}
+
+template <typename T, typename M, typename Out, int ZC, int CXM>
+static hipError_t emit_mc4_z(bool vec, const VolArgs& a, const VolArgs* vols,
+                             const int32_t* emap, int64_t gbase, int64_t nblocks,
+                             hipStream_t st) {
+  if (vols) {
+    if (vec)
+      hipLaunchKernelGGL((k_emit_mc4_b<T, M, Out, ZC, true, CXM>), dim3((unsigned)nblocks),
+                         dim3(kBlock), 0, st, vols, emap, gbase);
+    else
+      hipLaunchKernelGGL((k_emit_mc4_b<T, M, Out, ZC, false, CXM>), dim3((unsigned)nblocks),
+                         dim3(kBlock), 0, st, vols, emap, gbase);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, true, CXM>), dim3((unsigned)a.ntiles),
+                         dim3(kBlock), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, false, CXM>), dim3((unsigned)a.ntiles),
+                         dim3(kBlock), 0, st, a);
+  }
+  return hipGetLastError();
+}
+
+template <typename T, typename M, typename Out, int ZC>
+static hipError_t emit_mc4_c(int variant, const VolArgs& a, const VolArgs* vols,
+                             const int32_t* emap, int64_t gbase, int64_t nblocks,
+                             hipStream_t st) {
+  const bool vec = (variant & 1) == 0;
+  switch (variant >> 2) {
+    case 1: return emit_mc4_z<T, M, Out, ZC, 0>(vec, a, vols, emap, gbase, nblocks, st);
+    case 2: return emit_mc4_z<T, M, Out, ZC, 7>(vec, a, vols, emap, gbase, nblocks, st);
+    default: return emit_mc4_z<T, M, Out, ZC, -1>(vec, a, vols, emap, gbase, nblocks, st);
+  }
 }
 
 template <typename T, typename M, typename Out>
-static hipError_t emit_mc4_t(const EmitArgs& a, void* verts, int64_t* faces, hipStream_t st) {
-  constexpr int ZC = sizeof(T) == 4 ? 32 : 16;
-  // the brick's DMA offsets are 32-bit and span ZC+1 planes
-  if ((int64_t)(ZC + 1) * a.g.nx * a.g.ny * (int64_t)sizeof(T) >= (1ll << 31))
-    return hipErrorNotSupported;
-  EmitArgs b = a;
-  b.nti = (a.g.nx - 1 + kEmitTX - 1) / kEmitTX;
-  b.ntj = a.g.ny - 1;
-  b.ntiles = (int64_t)b.nti * b.ntj * a.g.nzseg;
-  // 16-byte DMA needs 16-byte aligned sample rows
-  const bool vec = ((uintptr_t)a.f % 16 == 0) && (((int64_t)a.g.nx * (int64_t)sizeof(T)) % 16 == 0);
-  if (vec)
-    hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, true>), dim3((unsigned)b.ntiles), dim3(kBlock),
-                       0, st, b, (Out*)verts, faces);
-  else
-    hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, false>), dim3((unsigned)b.ntiles), dim3(kBlock),
-                       0, st, b, (Out*)verts, faces);
-  return hipGetLastError();
+static hipError_t emit_mc4_t(int variant, const VolArgs& a, const VolArgs* vols,
+                             const int32_t* emap, int64_t gbase, int64_t nblocks,
+                             hipStream_t st) {
+  if constexpr (sizeof(T) == 4) {
+    if (variant & 2) return emit_mc4_c<T, M, Out, 16>(variant, a, vols, emap, gbase, nblocks, st);
+    return emit_mc4_c<T, M, Out, 32>(variant, a, vols, emap, gbase, nblocks, st);
+  } else {
+    return emit_mc4_c<T, M, Out, 16>(variant, a, vols, emap, gbase, nblocks, st);
+  }
 }
 
-hipError_t launch_emit_mc4(const Plan& p, const EmitArgs& a, void* verts, int64_t* faces,
-                           hipStream_t st) {
+static hipError_t emit_mc4_any(const Plan& p, int variant, const VolArgs& a, const VolArgs* vols,
+                               const int32_t* emap, int64_t gbase, int64_t nblocks,
+                               hipStream_t st) {
   if (p.T == K_F32 && p.M == K_F32 && p.Out == K_F32)
-    return emit_mc4_t<float, float, float>(a, verts, faces, st);
-  if (p.T == K_F32 && p.M == K_F32) return emit_mc4_t<float, float, double>(a, verts, faces, st);
-  if (p.T == K_F32) return emit_mc4_t<float, double, double>(a, verts, faces, st);
-  return emit_mc4_t<double, double, double>(a, verts, faces, st);
+    return emit_mc4_t<float, float, float>(variant, a, vols, emap, gbase, nblocks, st);
+  if (p.T == K_F32 && p.M == K_F32)
+    return emit_mc4_t<float, float, double>(variant, a, vols, emap, gbase, nblocks, st);
+  if (p.T == K_F32) return emit_mc4_t<float, double, double>(variant, a, vols, emap, gbase, nblocks, st);
+  return emit_mc4_t<double, double, double>(variant, a, vols, emap, gbase, nblocks, st);
+}
+
+hipError_t launch_emit_mc4(const Plan& p, int variant, const VolArgs& a, hipStream_t st) {
+  if (a.ntiles == 0) return hipSuccess;
+  return emit_mc4_any(p, variant, a, nullptr, nullptr, 0, 0, st);
+}
+
+hipError_t launch_emit_mc4_batch(const Plan& p, int variant, const VolArgs* vols,
+                                 const int32_t* emap, int64_t gbase, int64_t nblocks,
+                                 hipStream_t st) {
+  if (nblocks == 0) return hipSuccess;
+  return emit_mc4_any(p, variant, VolArgs{}, vols, emap, gbase, nblocks, st);
 }
 
 }  // namespace msh

@@ -23,6 +23,11 @@
 //   -ffp-contract=off (Julia never fuses a*b+c unless asked).
 // Naive Surface Nets has no code in v0.7.0; its definition here is this
 // project's own (see sn_isosurface below).
+//
+// Parity unpinned (no Julia run is possible here): the interior rounding of
+// LinRange (only its endpoints, symmetry and the exact values of Julia's
+// LinRange docstring are checked), and the noisy-sphere counts of
+// test/runtests.jl:152-172, which depend on Julia's MersenneTwister stream.
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
