@@ -99,11 +99,11 @@ class Runner:
         self.extract()
 
 
-def measure(m, dtype, n, method_name, steps, warmup):
+def measure(m, dtype, n, method_name, steps, warmup, kind="value_noise"):
     import torch
     meth = {"mc": m.MarchingCubes(iso=0), "mt": m.MarchingTetrahedra(iso=0),
             "sn": m.NaiveSurfaceNets(iso=0)}[method_name]
-    field = m.generate_field("value_noise", (n, n, n), dtype, seed=1)
+    field = m.generate_field(kind, (n, n, n), dtype, seed=1)
     torch.cuda.synchronize()
     r = Runner(m, field, meth)
     for _ in range(warmup):
@@ -378,6 +378,14 @@ def main():
     f64 = measure(m, "float64", a.n, a.method, a.steps, a.warmup)
     f32 = measure(m, "float32", a.n, a.method, a.steps, a.warmup)
     main_r = f64
+    gyroid = {}
+    for meth in ("mt", "sn", "mc"):
+        r = measure(m, "float32", 256, meth, a.steps, a.warmup, kind="gyroid")
+        gyroid[meth] = dict(gvoxels_s=round(r["gvox_s"], 3), tris_s=round(r["tris_s"], 1),
+                            ms_per_step=round(r["step_s"] * 1e3, 4),
+                            count_ms=round(r["count_s"] * 1e3, 4),
+                            emit_ms=round(r["emit_s"] * 1e3, 4),
+                            nverts=r["nverts"], nfaces=r["nfaces"])
     batch = measure_batch(m, a.batch_volumes, max(2, min(a.steps, 5))) if a.batch_volumes else None
     serve = (measure_serve(m, a.serve_requests, [float(q) for q in a.serve_qps.split(",")])
              if a.serve_requests else None)
@@ -423,6 +431,7 @@ def main():
         "roofline": roofline(main_r),
         "cpu_baseline": cpu,
         "hbm_copy_probe": copy,
+        "gyroid_256_f32": gyroid,
         "batched": batch,
         "serving": serve,
         "float64": summary(f64),

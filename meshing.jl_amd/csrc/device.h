@@ -165,7 +165,8 @@ hipError_t launch_count4(const Plan& p, const VolArgs& a, hipStream_t st);
 hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
                                int64_t gbase, int64_t nblocks, hipStream_t st);
 // -1: volume too large for k_emit_mc4; else bit 0: dword DMA (not 16-byte),
-// bit 1: 16-cell z chunks for Float32.  Fills the emit tiling.
+// bits 1-2: axis types (1 all Float64, 2 all Float32, 0 mixed), bits 3-4:
+// tile shape.  Fills the emit tiling.
 int mc4_variant(const Plan& p, VolArgs* a);
 hipError_t launch_emit_mc4(const Plan& p, int variant, const VolArgs& a, hipStream_t st);
 hipError_t launch_emit_mc4_batch(const Plan& p, int variant, const VolArgs* vols,

@@ -806,8 +806,7 @@ static int generic_shape(const Plan& p) {
 
 static const int kShapeTX[4] = {31, 31, 15, 15}, kShapeTY[4] = {2, 1, 2, 1};
 
-// Variants: 0..11 = k_emit_mc4 (bit 0 dword DMA, bit 1 16-cell chunks, bits
-// 2-3 axis types: 1 all Float64, 2 all Float32, 0 mixed); 16 + shape = k_emit.
+// Variants: 0..31 = k_emit_mc4 (see mc4_variant); 32 + shape = k_emit.
 int emit_variant(const Plan& p, VolArgs* a) {
   if (p.method == MC) {
     const char* legacy = getenv("MSH_EMIT_V3");
@@ -820,7 +819,7 @@ int emit_variant(const Plan& p, VolArgs* a) {
   a->nti = (a->g.nx - 1 + kShapeTX[sh] - 1) / kShapeTX[sh];
   a->ntj = (a->g.ny - 1 + kShapeTY[sh] - 1) / kShapeTY[sh];
   a->ntiles = (a->g.nx > 1 && a->g.ny > 1) ? (int64_t)a->nti * a->ntj * a->g.nzseg : 0;
-  return 16 + sh;
+  return 32 + sh;
 }
 
 void count_tiling(const Plan& p, VolArgs* a) { count4_tiling(p, a); }
@@ -900,13 +899,13 @@ static hipError_t emit_types(const Plan& p, int sh, const VolArgs& a, const VolA
 
 static hipError_t emit_any(const Plan& p, int variant, const VolArgs& a, const VolArgs* vols,
                            const int32_t* emap, int64_t gbase, int64_t nblocks, hipStream_t st) {
-  if (variant < 16) {
+  if (variant < 32) {
     if (vols) return launch_emit_mc4_batch(p, variant, vols, emap, gbase, nblocks, st);
     return launch_emit_mc4(p, variant, a, st);
   }
   if (!vols && a.ntiles == 0) return hipSuccess;
   if (vols && nblocks == 0) return hipSuccess;
-  const int sh = variant - 16;
+  const int sh = variant - 32;
   Plan q = p;
   if (p.method != MT) q.A = q.M;  // eps is used by Marching Tetrahedra only
   if (p.method == MC) return emit_types<MC>(q, sh, a, vols, emap, gbase, nblocks, st);

@@ -7,6 +7,10 @@
 // face are pulled out of a mask with one shift.  Bits are kept in
 // "natural" order inside each nibble -- (x,y) (x+1,y) (x,y+1) (x+1,y+1) --
 // and the lookup tables are permuted once per block to that order.
+#include <stdlib.h>
+
+#include <type_traits>
+
 #include "device.h"
 
 #define MSH_LUT static __constant__
@@ -229,14 +233,16 @@ __global__ __launch_bounds__(kBlock) void k_count4_b(const VolArgs* __restrict__
 //    slot o), finds its cell by binary search over the list prefixes, and
 //    writes output rbv[row] + slot -- consecutive lanes write consecutive
 //    outputs of a row.
-template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM>
+template <typename T, typename M, typename Out, int ZC, int CY, bool VEC, int CXM>
 __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
-  constexpr int TX = kEmitTX, SX = 64, SY = 2;
+  constexpr int TX = kEmitTX, SX = 64, SY = CY + 1;
   constexpr int R = kWave / ZC;
   constexpr int NP = ZC + 1;
   constexpr int DW = (int)(sizeof(T) / 4);
   constexpr int RD = SX * DW;  // dwords per sample row
-  constexpr int NSTEP = (TX + R - 1) / R;
+  constexpr int NRW = TX * CY;  // cell rows (x, y) of the tile
+  static_assert(TX % R == 0, "a wave step must not straddle two y rows");
+  constexpr int NSTEP = NRW / R;
   constexpr int CAP = 128;
   constexpr int VW = CAP * 12 / 64, FW = CAP * 5 / 64;  // slot bitmap words
   struct Smem {
@@ -246,8 +252,8 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
     uint64_t mask[NP * SY];  // [plane][y], bit x
     uint32_t cnt[256];
     uint64_t lv[256], lt[256];
-    uint8_t le[16];
-    int64_t rbv[2][SX], rbf[2][SX];
+    uint32_t le[16];  // per edge: brick offsets of its corners a, b and their (dx,dy,dz)
+    int64_t rbv[2][NRW], rbf[2][NRW];  // output offset of each row at this / next chunk
     uint64_t list[kWaves][CAP];
     uint16_t lpv[kWaves][CAP], lpf[kWaves][CAP];
     uint64_t vbits[kWaves][VW], fbits[kWaves][FW];  // bit o: slot o starts an entry
@@ -258,7 +264,7 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
   int64_t* __restrict__ faces = a.faces;
   const Geom g = a.g;
   const int lane = lane_id(), wv = wave_id();
-  const int jb = (int)(t % a.ntj);
+  const int jb = (int)(t % a.ntj) * CY;
   const int64_t rest = t / a.ntj;
   const int ib = (int)(rest % a.nti) * TX;
   const int s = (int)(rest / a.nti);
@@ -281,18 +287,30 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
     sm.lv[c] = vw;
     sm.lt[c] = tw;
   }
-  if (threadIdx.x < 12) sm.le[threadIdx.x] = MC_EDGE[threadIdx.x];
-  for (int c = threadIdx.x; c < SX; c += kBlock) {
+  if (threadIdx.x < 12) {
+    // edge e runs from corner a to corner b (src/lut/mc.jl _mc_edge_list);
+    // word: offset of a (dwords from the cell's corner 0 in the brick) |
+    // offset of b << 10 | dx,dy,dz of a << 20 | dx,dy,dz of b << 23
+    const uint32_t ew = MC_EDGE[threadIdx.x];
+    const int ca = ew & 0xF, cb = ew >> 4;
+    const uint32_t oa = (uint32_t)((mc_dz(ca) * SY + mc_dy(ca)) * RD + mc_dx(ca) * DW);
+    const uint32_t ob = (uint32_t)((mc_dz(cb) * SY + mc_dy(cb)) * RD + mc_dx(cb) * DW);
+    sm.le[threadIdx.x] = oa | (ob << 10) |
+                         ((uint32_t)(mc_dx(ca) | (mc_dy(ca) << 1) | (mc_dz(ca) << 2)) << 20) |
+                         ((uint32_t)(mc_dx(cb) | (mc_dy(cb) << 1) | (mc_dz(cb) << 2)) << 23);
+  }
+  for (int rr = threadIdx.x; rr < NRW; rr += kBlock) {
+    const int ci = rr % TX, cj = rr / TX;
     int64_t bv = 0, bf = 0;
-    if (c < TX && ib + c < g.nx - 1) {
-      const int64_t u = a.ubase + ((int64_t)(ib + c) * (g.ny - 1) + jb) * g.nzseg + s;
+    if (ib + ci < g.nx - 1 && jb + cj < g.ny - 1) {
+      const int64_t u = a.ubase + ((int64_t)(ib + ci) * (g.ny - 1) + jb + cj) * g.nzseg + s;
       bv = a.voff[u] - vbase;
       bf = a.foff[u] - fbase;
     }
-    sm.rbv[0][c] = bv;
-    sm.rbf[0][c] = bf;
-    sm.cx[c] = a.coords[min(ib + c, g.nx - 1)];
+    sm.rbv[0][rr] = bv;
+    sm.rbf[0][rr] = bf;
   }
+  for (int c = threadIdx.x; c < SX; c += kBlock) sm.cx[c] = a.coords[min(ib + c, g.nx - 1)];
   if (threadIdx.x < SY) sm.cy[threadIdx.x] = a.coords[g.nx + min(jb + (int)threadIdx.x, g.ny - 1)];
 
   // DMA layout.  VEC: 16 bytes per lane, so one instruction fills 1 KiB of
@@ -374,10 +392,6 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
     }
     lds_barrier();
 
-    auto sample = [&](int c, int ci, int corner) -> T {
-      return *(const T*)(smp + ((c + mc_dz(corner)) * SY + mc_dy(corner)) * RD +
-                         (ci + mc_dx(corner)) * DW);
-    };
     int n = 0;
     uint32_t V = 0, F = 0;
     auto flush = [&]() {
@@ -392,18 +406,18 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
         before += __popcll(word);
         if (o >= V) break;
         const uint64_t e = list[k];
-        const int ci = (int)(e & 63), c = (int)((e >> 6) & 31);
-        const uint32_t cs = (uint32_t)((e >> 11) & 0xFF), pv = (uint32_t)((e >> 19) & 0x1FF);
+        const int ci = (int)(e & 63), cj = (int)((e >> 6) & 3), c = (int)((e >> 8) & 31);
+        const uint32_t cs = (uint32_t)((e >> 13) & 0xFF), pv = (uint32_t)((e >> 21) & 0x1FF);
         const uint32_t q = o - lpv[k];
         const uint32_t ed = (uint32_t)(sm.lv[cs] >> (4 + 4 * q)) & 0xF;
         const uint32_t ew = sm.le[ed];
-        const int ca = ew & 0xF, cb = ew >> 4;
-        const T va = sample(c, ci, ca), vb = sample(c, ci, cb);
+        const uint32_t* cell = smp + c * (SY * RD) + cj * RD + ci * DW;
+        const T va = *(const T*)(cell + (ew & 0x3FF)), vb = *(const T*)(cell + ((ew >> 10) & 0x3FF));
         const M mu = (iso - (M)va) / (M)(vb - va);
-        const double px = lerp_axis<M>(cx32, mu, sm.cx[ci + mc_dx(ca)], sm.cx[ci + mc_dx(cb)]);
-        const double py = lerp_axis<M>(cy32, mu, sm.cy[mc_dy(ca)], sm.cy[mc_dy(cb)]);
-        const double pz = lerp_axis<M>(cz32, mu, czs[c + mc_dz(ca)], czs[c + mc_dz(cb)]);
-        const int64_t vi = rbv[ci] + pv + q;
+        const double px = lerp_axis<M>(cx32, mu, sm.cx[ci + ((ew >> 20) & 1)], sm.cx[ci + ((ew >> 23) & 1)]);
+        const double py = lerp_axis<M>(cy32, mu, sm.cy[cj + ((ew >> 21) & 1)], sm.cy[cj + ((ew >> 24) & 1)]);
+        const double pz = lerp_axis<M>(cz32, mu, czs[c + ((ew >> 22) & 1)], czs[c + ((ew >> 25) & 1)]);
+        const int64_t vi = rbv[cj * TX + ci] + pv + q;
         if (vi < totv) {
           Out* out = verts + vi * 3;
           out[0] = (Out)px;
@@ -420,13 +434,13 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
         before += __popcll(word);
         if (o >= F) break;
         const uint64_t e = list[k];
-        const int ci = (int)(e & 63);
-        const uint32_t cs = (uint32_t)((e >> 11) & 0xFF), pv = (uint32_t)((e >> 19) & 0x1FF),
-                       pf = (uint32_t)((e >> 28) & 0x1FF);
+        const int rr = (int)((e >> 6) & 3) * TX + (int)(e & 63);
+        const uint32_t cs = (uint32_t)((e >> 13) & 0xFF), pv = (uint32_t)((e >> 21) & 0x1FF),
+                       pf = (uint32_t)((e >> 30) & 0x1FF);
         const uint32_t q = o - lpf[k];
         const uint64_t w = sm.lt[cs] >> (3 + 12 * q);
-        const int64_t vb = rbv[ci] + pv + 1;  // 1-based
-        const int64_t fi = rbf[ci] + pf + q;
+        const int64_t vb = rbv[rr] + pv + 1;  // 1-based
+        const int64_t fi = rbf[rr] + pf + q;
         if (fi < totf) {
           int64_t* out = faces + fi * 3;
           out[0] = vb + (int64_t)(w & 0xF);
@@ -442,9 +456,10 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
     const int kc = zc + z;
     const bool zvalid = kc < ze;
     for (int st = wv; st < NSTEP; st += kWaves) {
-      const int ci = st * R + h;
-      const bool valid = zvalid && ci < TX && ib + ci < g.nx - 1;
-      const uint64_t* mz = sm.mask + z * SY;
+      const int r0 = st * R, cj = r0 / TX;  // the step's y row (wave-uniform)
+      const int rr = r0 + h, ci = r0 - cj * TX + h;
+      const bool valid = zvalid && ib + ci < g.nx - 1 && jb + cj < g.ny - 1;
+      const uint64_t* mz = sm.mask + z * SY + cj;
       const uint32_t lo = (uint32_t)((mz[0] >> ci) & 3) | ((uint32_t)((mz[1] >> ci) & 3) << 2);
       const uint32_t hi =
           (uint32_t)((mz[SY] >> ci) & 3) | ((uint32_t)((mz[SY + 1] >> ci) & 3) << 2);
@@ -462,9 +477,9 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
         if (r == h) mine = tots[r];
         sum += tots[r];
       }
-      if (z == 0 && ci < SX) {
-        rbv_next[ci] = rbv[ci] + (mine & 0xFFFF);
-        rbf_next[ci] = rbf[ci] + (mine >> 16);
+      if (z == 0) {
+        rbv_next[rr] = rbv[rr] + (mine & 0xFFFF);
+        rbf_next[rr] = rbf[rr] + (mine >> 16);
       }
       if (sum == 0) continue;
       const uint32_t ex_row = inc - packed;   // prefix inside the cell's row
@@ -482,8 +497,8 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
       }
       if (active) {
         const int k = n + (int)lanes_below(bal);
-        list[k] = (uint64_t)ci | ((uint64_t)z << 6) | ((uint64_t)cs << 11) |
-                  ((uint64_t)(ex_row & 0xFFFF) << 19) | ((uint64_t)(ex_row >> 16) << 28);
+        list[k] = (uint64_t)(ci | (cj << 6)) | ((uint64_t)z << 8) | ((uint64_t)cs << 13) |
+                  ((uint64_t)(ex_row & 0xFFFF) << 21) | ((uint64_t)(ex_row >> 16) << 30);
         const uint32_t sv = V + (ex & 0xFFFF), sf = F + (ex >> 16);
         lpv[k] = (uint16_t)sv;
         lpf[k] = (uint16_t)sf;
@@ -503,18 +518,18 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
   }
 }
 
-template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM>
+template <typename T, typename M, typename Out, int ZC, int CY, bool VEC, int CXM>
 __global__ __launch_bounds__(kBlock) void k_emit_mc4(VolArgs a) {
-  emit_mc4_tile<T, M, Out, ZC, VEC, CXM>(a, xcd_tile(blockIdx.x, a.ntiles));
+  emit_mc4_tile<T, M, Out, ZC, CY, VEC, CXM>(a, xcd_tile(blockIdx.x, a.ntiles));
 }
 
-template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM>
+template <typename T, typename M, typename Out, int ZC, int CY, bool VEC, int CXM>
 __global__ __launch_bounds__(kBlock) void k_emit_mc4_b(const VolArgs* __restrict__ vols,
                                                        const int32_t* __restrict__ emap,
                                                        int64_t gbase) {
   const int64_t b = gbase + blockIdx.x;
   const VolArgs a = vols[emap[b]];
-  emit_mc4_tile<T, M, Out, ZC, VEC, CXM>(a, b - a.eblk0);
+  emit_mc4_tile<T, M, Out, ZC, CY, VEC, CXM>(a, b - a.eblk0);
 }
 
 }  // namespace
@@ -576,20 +591,23 @@ hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t
   return count4_any(p, VolArgs{}, vols, cmap, gbase, nblocks, st);
 }
 
-// z chunk of the Marching Cubes emit: 32 (Float32) or 16 cells.  Float32
-// also has a 16-cell form with half the LDS brick; MSH_MC4_ZC=16 selects it
-// (for tuning).
-static int mc4_zc(int T) {
-  if (T != K_F32) return 16;
-  static const int zc = [] {
-    const char* e = getenv("MSH_MC4_ZC");
-    return (e && atoi(e) == 16) ? 16 : 32;
+// Tile shape of the Marching Cubes emit: z chunks of 16 cells and one cell
+// row along y per block (on 1024^3 value noise this beat 32-cell chunks,
+// whose larger LDS brick leaves fewer blocks per CU, and two y rows per
+// block, likewise).  MSH_MC4_SHAPE=1 selects 32-cell chunks for the
+// all-Float32 instantiation with Float64 axes (tuning).
+static int mc4_shape() {
+  static const int sh = [] {
+    const char* e = getenv("MSH_MC4_SHAPE");
+    return (e && atoi(e) == 1) ? 1 : 0;
   }();
-  return zc;
+  return sh;
 }
 
 int mc4_variant(const Plan& p, VolArgs* a) {
-  const int zc = mc4_zc(p.T);
+  const bool tunable = p.T == K_F32 && p.M == K_F32 && p.Out == K_F32 && a->cxmask == 0;
+  const int sh = tunable ? mc4_shape() : 0;
+  const int zc = sh ? 32 : 16;
   const int64_t esz = p.T == K_F32 ? 4 : 8;
   // the brick's DMA offsets are 32-bit and span ZC+1 planes
   if ((int64_t)(zc + 1) * a->g.nx * a->g.ny * esz >= (1ll << 31)) return -1;
@@ -599,53 +617,45 @@ int mc4_variant(const Plan& p, VolArgs* a) {
   // 16-byte DMA needs 16-byte aligned sample rows
   const bool vec = ((uintptr_t)a->f % 16 == 0) && (((int64_t)a->g.nx * esz) % 16 == 0);
   const int cxclass = a->cxmask == 0 ? 1 : a->cxmask == 7 ? 2 : 0;
-  return (vec ? 0 : 1) | (zc == 16 && p.T == K_F32 ? 2 : 0) | (cxclass << 2);
+  return (vec ? 0 : 1) | (cxclass << 1) | (sh << 3);
 }
 
-template <typename T, typename M, typename Out, int ZC, int CXM>
+template <typename T, typename M, typename Out, int ZC, int CY, int CXM>
 static hipError_t emit_mc4_z(bool vec, const VolArgs& a, const VolArgs* vols,
                              const int32_t* emap, int64_t gbase, int64_t nblocks,
                              hipStream_t st) {
   if (vols) {
     if (vec)
-      hipLaunchKernelGGL((k_emit_mc4_b<T, M, Out, ZC, true, CXM>), dim3((unsigned)nblocks),
+      hipLaunchKernelGGL((k_emit_mc4_b<T, M, Out, ZC, CY, true, CXM>), dim3((unsigned)nblocks),
                          dim3(kBlock), 0, st, vols, emap, gbase);
     else
-      hipLaunchKernelGGL((k_emit_mc4_b<T, M, Out, ZC, false, CXM>), dim3((unsigned)nblocks),
+      hipLaunchKernelGGL((k_emit_mc4_b<T, M, Out, ZC, CY, false, CXM>), dim3((unsigned)nblocks),
                          dim3(kBlock), 0, st, vols, emap, gbase);
   } else {
     if (vec)
-      hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, true, CXM>), dim3((unsigned)a.ntiles),
+      hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, CY, true, CXM>), dim3((unsigned)a.ntiles),
                          dim3(kBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, false, CXM>), dim3((unsigned)a.ntiles),
+      hipLaunchKernelGGL((k_emit_mc4<T, M, Out, ZC, CY, false, CXM>), dim3((unsigned)a.ntiles),
                          dim3(kBlock), 0, st, a);
   }
   return hipGetLastError();
-}
-
-template <typename T, typename M, typename Out, int ZC>
-static hipError_t emit_mc4_c(int variant, const VolArgs& a, const VolArgs* vols,
-                             const int32_t* emap, int64_t gbase, int64_t nblocks,
-                             hipStream_t st) {
-  const bool vec = (variant & 1) == 0;
-  switch (variant >> 2) {
-    case 1: return emit_mc4_z<T, M, Out, ZC, 0>(vec, a, vols, emap, gbase, nblocks, st);
-    case 2: return emit_mc4_z<T, M, Out, ZC, 7>(vec, a, vols, emap, gbase, nblocks, st);
-    default: return emit_mc4_z<T, M, Out, ZC, -1>(vec, a, vols, emap, gbase, nblocks, st);
-  }
 }
 
 template <typename T, typename M, typename Out>
 static hipError_t emit_mc4_t(int variant, const VolArgs& a, const VolArgs* vols,
                              const int32_t* emap, int64_t gbase, int64_t nblocks,
                              hipStream_t st) {
-  if constexpr (sizeof(T) == 4) {
-    if (variant & 2) return emit_mc4_c<T, M, Out, 16>(variant, a, vols, emap, gbase, nblocks, st);
-    return emit_mc4_c<T, M, Out, 32>(variant, a, vols, emap, gbase, nblocks, st);
-  } else {
-    return emit_mc4_c<T, M, Out, 16>(variant, a, vols, emap, gbase, nblocks, st);
+  const bool vec = (variant & 1) == 0;
+  const int cxclass = (variant >> 1) & 3, sh = variant >> 3;
+#define MSH_ARGS vec, a, vols, emap, gbase, nblocks, st
+  if constexpr (sizeof(T) == 4 && std::is_same<T, M>::value && std::is_same<M, Out>::value) {
+    if (cxclass == 1 && sh == 1) return emit_mc4_z<T, M, Out, 32, 1, 0>(MSH_ARGS);
   }
+  if (cxclass == 1) return emit_mc4_z<T, M, Out, 16, 1, 0>(MSH_ARGS);
+  if (cxclass == 2) return emit_mc4_z<T, M, Out, 16, 1, 7>(MSH_ARGS);
+  return emit_mc4_z<T, M, Out, 16, 1, -1>(MSH_ARGS);
+#undef MSH_ARGS
 }
 
 static hipError_t emit_mc4_any(const Plan& p, int variant, const VolArgs& a, const VolArgs* vols,

@@ -101,7 +101,10 @@ class Server:
         self.running = True
         self.done_q = collections.deque()
         self.done_cv = threading.Condition()
-        self.stats = dict(batches=0, requests=0, max_batch_seen=0)
+        # host seconds spent per phase (count + size read-back, output
+        # allocation, emit launch, waiting for completion)
+        self.stats = dict(batches=0, requests=0, max_batch_seen=0, t_count=0.0, t_alloc=0.0,
+                          t_extract=0.0, t_wait=0.0)
         self.sched = threading.Thread(target=self._schedule, name="msh-sched", daemon=True)
         self.compl = threading.Thread(target=self._complete, name="msh-complete", daemon=True)
         self.sched.start()
@@ -200,7 +203,9 @@ class Server:
                     P = (MshParams * n)(*[_params(r.method, *r.axes) for r in batch])
                     nv = (ctypes.c_int64 * n)()
                     nf = (ctypes.c_int64 * n)()
+                    t0 = time.perf_counter()
                     ctx.check(self.L.msh_batch_count(ctx.handle, n, F, P, nv, nf))
+                    t1 = time.perf_counter()
                     vp = (ctypes.c_void_p * n)()
                     fp = (ctypes.c_void_p * n)()
                     extra = 0
@@ -220,9 +225,14 @@ class Server:
                         r.reserved = exact
                     with self.lock:
                         self.in_use += extra
+                    t2 = time.perf_counter()
                     ctx.check(self.L.msh_batch_extract(ctx.handle, n, F, P, vp, fp, MSH_DEVICE))
                     ev = torch.cuda.Event()
                     ev.record(stream)
+                    t3 = time.perf_counter()
+                    self.stats["t_count"] += t1 - t0
+                    self.stats["t_alloc"] += t2 - t1
+                    self.stats["t_extract"] += t3 - t2
             except BaseException as e:  # hand the error to every request of the batch
                 for r in batch:
                     r.future.set_exception(e)
@@ -247,8 +257,10 @@ class Server:
             if item is None:
                 return
             ev, batch = item
+            tw = time.perf_counter()
             ev.synchronize()
             t = time.perf_counter()
+            self.stats["t_wait"] += t - tw
             freed = 0
             for r in batch:
                 r.t_done = t
