@@ -258,27 +258,53 @@ def serve_workload(m, nreq, seed):
     return out
 
 
+def plain_request(m, field, method, capi, L, ctx):
+    """One plain extraction the way a caller without batching runs it:
+    count, allocate the outputs, extract."""
+    import torch
+    f = m._field(field)[0]
+    p = m._params(method, (-1, 1), (-1, 1), (-1, 1))
+    nv, nf, ok = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+    ctx.check(L.msh_count(ctx.handle, ctypes.byref(f), ctypes.byref(p), ctypes.byref(nv),
+                          ctypes.byref(nf)))
+    ctx.check(L.msh_output_kind(ctypes.byref(f), ctypes.byref(p), ctypes.byref(ok)))
+    dt = torch.float32 if ok.value == capi.MSH_FLOAT32 else torch.float64
+    V = torch.empty((nv.value, 3), dtype=dt, device=field.device)
+    F = torch.empty((nf.value, 3), dtype=torch.int64, device=field.device)
+    ctx.check(L.msh_extract(ctx.handle, ctypes.byref(f), ctypes.byref(p),
+                            ctypes.c_void_p(V.data_ptr()), ctypes.c_void_p(F.data_ptr()),
+                            capi.MSH_DEVICE))
+    return V, F
+
+
 def measure_serve(m, nreq, qps_list, seed=11):
-    """Plain sequential extraction of the request set, then the continuous-
-    batching server under Poisson arrivals at each offered rate."""
+    """Plain sequential extraction of the request set (count, allocate,
+    extract, hand the result to the client, which drops it), then the
+    continuous-batching server under Poisson arrivals at each offered rate;
+    the client takes each result in a completion callback and drops it."""
     import numpy as np
     import torch
+    from importlib import import_module
+    capi = import_module(m.__name__ + "._capi")
+    L = capi.lib()
     reqs = serve_workload(m, nreq, seed)
     torch.cuda.synchronize()
     samples = sum(f.numel() for f, _ in reqs)
-    plain = [Runner(m, f, me) for f, me in reqs]
-    for r in plain:
-        r.step()
+    ctx = capi.context(reqs[0][0].device.index)
+    ctx.check(L.msh_set_stream(ctx.handle,
+                               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    for f, me in reqs:  # warm-up: workspace and the allocator's cache
+        plain_request(m, f, me, capi, L, ctx)
     torch.cuda.synchronize()
     lat = []
     t0 = time.perf_counter()
-    for r in plain:
+    for f, me in reqs:
         t1 = time.perf_counter()
-        r.step()
+        out = plain_request(m, f, me, capi, L, ctx)
         torch.cuda.synchronize()
+        del out
         lat.append(time.perf_counter() - t1)
     tp = time.perf_counter() - t0
-    del plain
     res = dict(requests=nreq, samples=samples, seed=seed,
                plain=dict(req_s=round(nreq / tp, 1), p50_ms=round(np.percentile(lat, 50) * 1e3, 3),
                           p99_ms=round(np.percentile(lat, 99) * 1e3, 3)),
@@ -286,9 +312,11 @@ def measure_serve(m, nreq, qps_list, seed=11):
     rng = np.random.default_rng(seed + 1)
     for qps in qps_list:
         gaps = rng.exponential(1.0 / qps, size=nreq) if qps else np.zeros(nreq)
+        lats = []
         with m.Server(hbm_budget=64 << 30, max_batch=128) as srv:
-            # warm the server's contexts and workspaces once
-            srv.submit(*reqs[0]).result()
+            for f, me in reqs[:8]:  # warm the server's contexts and workspaces
+                srv.submit(f, me).result()
+            base = dict(srv.stats)
             futs = []
             t0 = time.perf_counter()
             due = t0
@@ -297,17 +325,16 @@ def measure_serve(m, nreq, qps_list, seed=11):
                 d = due - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                futs.append(srv.submit(f, me))
+                futs.append(srv.submit(f, me, on_done=lambda r, l: lats.append(l)))
             for fu in futs:
                 fu.result()
             span = time.perf_counter() - t0
-            lats = [fu.latency for fu in futs]
-            st = dict(srv.stats)
+            st = {k: srv.stats[k] - base[k] for k in ("batches", "requests")}
         res["served"].append(dict(offered_qps=qps or "all at t=0", req_s=round(nreq / span, 1),
                                   p50_ms=round(np.percentile(lats, 50) * 1e3, 3),
                                   p99_ms=round(np.percentile(lats, 99) * 1e3, 3),
-                                  batches=st["batches"] - 1,
-                                  mean_batch=round((st["requests"] - 1) / max(st["batches"] - 1, 1), 2)))
+                                  batches=st["batches"],
+                                  mean_batch=round(st["requests"] / max(st["batches"], 1), 2)))
     best = max(x["req_s"] for x in res["served"])
     res["served_over_plain"] = round(best / res["plain"]["req_s"], 3)
     del reqs

@@ -159,6 +159,30 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uin
   return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
 }
 
+// Natural nibble order -> Marching Cubes corner order (src/marching_cubes.jl:42-49):
+// swap bits 2 and 3 of each nibble.
+__device__ __forceinline__ uint32_t nat_to_mc(uint32_t c) {
+  return (c & 0x33u) | ((c & 0x44u) << 1) | ((c & 0x88u) >> 1);
+}
+// Natural nibble order -> Marching Tetrahedra corner order
+// (src/marching_tetrahedra.jl:146-153): (x,y) (x,y+1) (x+1,y+1) (x+1,y).
+__device__ __forceinline__ uint32_t nat_to_mt(uint32_t c) {
+  return (c & 0x11u) | ((c & 0x44u) >> 1) | ((c & 0x88u) >> 1) | ((c & 0x22u) << 2);
+}
+
+// One LDS DMA instruction: each lane copies BYTES bytes from the buffer at
+// byte offset `off` to lds + lane*BYTES.  (The 16-byte form exists only in
+// the gfx950 device compile; the host compile never runs kernels.)
+template <int BYTES>
+__device__ __forceinline__ void dma_to_lds(__amdgpu_buffer_rsrc_t r, uint32_t* lds, uint32_t off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (BYTES == 16)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (void*)lds, 16, off, 0, 0, 0);
+  else
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (void*)lds, 4, off, 0, 0, 0);
+#endif
+}
+
 // lean.hip: the count kernel of every method and the Marching Cubes emit kernel.
 void count4_tiling(const Plan& p, VolArgs* a);
 hipError_t launch_count4(const Plan& p, const VolArgs& a, hipStream_t st);
@@ -169,6 +193,14 @@ hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t
 // tile shape.  Fills the emit tiling.
 int mc4_variant(const Plan& p, VolArgs* a);
 hipError_t launch_emit_mc4(const Plan& p, int variant, const VolArgs& a, hipStream_t st);
+// tet.hip: the Marching Tetrahedra and Naive Surface Nets emit kernel.  -1:
+// volume too large; else bit 0: dword DMA, bits 1-2: axis types.  Fills the
+// emit tiling.
+int halo_variant(const Plan& p, VolArgs* a);
+hipError_t launch_emit_halo(const Plan& p, int variant, const VolArgs& a, hipStream_t st);
+hipError_t launch_emit_halo_batch(const Plan& p, int variant, const VolArgs* vols,
+                                  const int32_t* emap, int64_t gbase, int64_t nblocks,
+                                  hipStream_t st);
 hipError_t launch_emit_mc4_batch(const Plan& p, int variant, const VolArgs* vols,
                                  const int32_t* emap, int64_t gbase, int64_t nblocks,
                                  hipStream_t st);

@@ -806,14 +806,19 @@ static int generic_shape(const Plan& p) {
 
 static const int kShapeTX[4] = {31, 31, 15, 15}, kShapeTY[4] = {2, 1, 2, 1};
 
-// Variants: 0..31 = k_emit_mc4 (see mc4_variant); 32 + shape = k_emit.
+// Variants: 0..31 = k_emit_mc4 (see mc4_variant); 32 + shape = k_emit;
+// 64 + v = k_emit_halo, tetrahedra and surface nets (see halo_variant).
+// MSH_EMIT_V3=1 selects k_emit for every method (comparison runs).
 int emit_variant(const Plan& p, VolArgs* a) {
-  if (p.method == MC) {
-    const char* legacy = getenv("MSH_EMIT_V3");
-    if (!(legacy && *legacy == '1')) {
-      const int v = mc4_variant(p, a);
-      if (v >= 0) return v;
-    }
+  const char* legacy = getenv("MSH_EMIT_V3");
+  const bool lean = !(legacy && *legacy == '1');
+  if (lean && p.method == MC) {
+    const int v = mc4_variant(p, a);
+    if (v >= 0) return v;
+  }
+  if (lean && (p.method == MT || p.method == SN)) {
+    const int v = halo_variant(p, a);
+    if (v >= 0) return 64 + v;
   }
   const int sh = generic_shape(p);
   a->nti = (a->g.nx - 1 + kShapeTX[sh] - 1) / kShapeTX[sh];
@@ -902,6 +907,10 @@ static hipError_t emit_any(const Plan& p, int variant, const VolArgs& a, const V
   if (variant < 32) {
     if (vols) return launch_emit_mc4_batch(p, variant, vols, emap, gbase, nblocks, st);
     return launch_emit_mc4(p, variant, a, st);
+  }
+  if (variant >= 64) {
+    if (vols) return launch_emit_halo_batch(p, variant - 64, vols, emap, gbase, nblocks, st);
+    return launch_emit_halo(p, variant - 64, a, st);
   }
   if (!vols && a.ntiles == 0) return hipSuccess;
   if (vols && nblocks == 0) return hipSuccess;

@@ -23,30 +23,6 @@ namespace {
 // aligned (when the row pitch is) while 61 samples fit a 64-lane row.
 constexpr int kEmitTX = 60;
 
-// Natural nibble order -> Marching Cubes corner order (src/marching_cubes.jl:42-49):
-// swap bits 2 and 3 of each nibble.
-__device__ __forceinline__ uint32_t nat_to_mc(uint32_t c) {
-  return (c & 0x33u) | ((c & 0x44u) << 1) | ((c & 0x88u) >> 1);
-}
-// Natural nibble order -> Marching Tetrahedra corner order
-// (src/marching_tetrahedra.jl:146-153): (x,y) (x,y+1) (x+1,y+1) (x+1,y).
-__device__ __forceinline__ uint32_t nat_to_mt(uint32_t c) {
-  return (c & 0x11u) | ((c & 0x44u) >> 1) | ((c & 0x88u) >> 1) | ((c & 0x22u) << 2);
-}
-
-// One LDS DMA instruction: each lane copies BYTES bytes from the buffer at
-// byte offset `off` to lds + lane*BYTES.  (The 16-byte form exists only in
-// the gfx950 device compile; the host compile never runs kernels.)
-template <int BYTES>
-__device__ __forceinline__ void dma_to_lds(__amdgpu_buffer_rsrc_t r, uint32_t* lds, uint32_t off) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  if constexpr (BYTES == 16)
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (void*)lds, 16, off, 0, 0, 0);
-  else
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (void*)lds, 4, off, 0, 0, 0);
-#endif
-}
-
 template <typename T>
 __device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
 template <>

@@ -111,8 +111,13 @@ class Server:
         self.compl.start()
 
     # -- client side ----------------------------------------------------
-    def submit(self, field, method, axes=None):
-        """Queue one extraction; returns a Future of (vertices, faces)."""
+    def submit(self, field, method, axes=None, on_done=None):
+        """Queue one extraction; returns a Future of (vertices, faces).
+
+        With ``on_done``, the completion thread calls ``on_done(result,
+        latency_s)`` instead and the future resolves to None, so the server
+        keeps no reference to the output once the callback returns (a
+        client that streams results out does not pile them up)."""
         if isinstance(field, FieldSpec):
             nbytes = field.nbytes
             nx, ny, nz = field.shape
@@ -124,6 +129,7 @@ class Server:
         units = max(nx - 1, 0) * max(ny - 1, 0)
         fut = Future()
         r = _Req(field, method, axes or ((-1, 1),) * 3, fut, time.perf_counter(), nbytes, units)
+        r.extra["on_done"] = on_done
         with self.lock:
             if not self.running:
                 raise RuntimeError("server is closed")
@@ -266,8 +272,18 @@ class Server:
                 r.t_done = t
                 r.extra.pop("field", None)
                 freed += r.reserved
-                r.future.set_result(r.result)
                 r.future.latency = t - r.t_submit
+                cb = r.extra.pop("on_done", None)
+                if cb is None:
+                    r.future.set_result(r.result)
+                else:
+                    try:
+                        cb(r.result, r.future.latency)
+                        r.future.set_result(None)
+                    except BaseException as e:
+                        r.future.set_exception(e)
+                r.result = None
+            batch = None
             with self.lock:
                 self.in_use -= freed
                 self.lock.notify_all()

@@ -86,3 +86,17 @@ def test_close_drains_queue(pkg):
     assert all(f.done() for f in futs)
     for (spec, m), f in zip(reqs, futs):
         same(f.result(), alone(pkg, spec, m))
+
+
+def test_on_done_callback_streams_results(pkg):
+    reqs = requests(pkg, 10, seed=21)
+    got = {}
+    with pkg.Server(max_batch=4) as srv:
+        futs = [srv.submit(s, m, on_done=lambda r, lat, i=i: got.__setitem__(i, (r, lat)))
+                for i, (s, m) in enumerate(reqs)]
+        assert all(f.result(timeout=60) is None for f in futs)
+    assert sorted(got) == list(range(len(reqs)))
+    for i, (spec, m) in enumerate(reqs):
+        r, lat = got[i]
+        assert lat >= 0
+        same(r, alone(pkg, spec, m))
