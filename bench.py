@@ -14,7 +14,7 @@ the same run and reported under "float32".
 
 Printed: one JSON line.  `value` is Gvoxels/s (samples per second / 1e9).
 `roofline` is for the dominant kernel (the Marching Cubes emit kernel,
-k_emit_mc4), from HIP events on the stream the library launches on; its
+k_emit_mc5), from HIP events on the stream the library launches on; its
 `traffic` is the HBM bytes per launch from the rocprofv3 FETCH_SIZE and
 WRITE_SIZE summaries kept in profiles/ (null when absent).  `cpu_baseline`
 times the CPU oracle (single thread) on an x-slab of the same field.  A
@@ -418,7 +418,7 @@ def main():
              if a.serve_requests else None)
     copy = copy_probe()
     cpu = None if a.no_cpu else cpu_baseline(m, a.n, a.cpu_planes)
-    kname = {"mc": "k_emit_mc4", "mt": "k_emit (tetrahedra)", "sn": "k_emit (surface nets)"}[a.method]
+    kname = {"mc": "k_emit_mc5", "mt": "k_emit_halo (tetrahedra)", "sn": "k_emit_halo (surface nets)"}[a.method]
 
     def roofline(r):
         b = r["field_bytes"] + r["out_bytes"]

@@ -193,6 +193,14 @@ hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t
 // tile shape.  Fills the emit tiling.
 int mc4_variant(const Plan& p, VolArgs* a);
 hipError_t launch_emit_mc4(const Plan& p, int variant, const VolArgs& a, hipStream_t st);
+// mc5.hip: the Marching Cubes emit kernel with lanes along x.  -1: volume
+// too large; else bit 0: dword DMA, bits 1-2: axis types.  Fills the tiling.
+int mc5_variant(const Plan& p, VolArgs* a);
+hipError_t launch_emit_mc5(const Plan& p, int variant, const VolArgs& a, hipStream_t st);
+hipError_t launch_emit_mc5_batch(const Plan& p, int variant, const VolArgs* vols,
+                                 const int32_t* emap, int64_t gbase, int64_t nblocks,
+                                 hipStream_t st);
+
 // tet.hip: the Marching Tetrahedra and Naive Surface Nets emit kernel.  -1:
 // volume too large; else bit 0: dword DMA, bits 1-2: axis types.  Fills the
 // emit tiling.

@@ -807,11 +807,17 @@ static int generic_shape(const Plan& p) {
 static const int kShapeTX[4] = {31, 31, 15, 15}, kShapeTY[4] = {2, 1, 2, 1};
 
 // Variants: 0..31 = k_emit_mc4 (see mc4_variant); 32 + shape = k_emit;
-// 64 + v = k_emit_halo, tetrahedra and surface nets (see halo_variant).
-// MSH_EMIT_V3=1 selects k_emit for every method (comparison runs).
+// 64 + v = k_emit_halo, tetrahedra and surface nets (see halo_variant);
+// 128 + v = k_emit_mc5 (see mc5_variant).  Comparison runs: MSH_EMIT_V3=1
+// selects k_emit for every method, MSH_MC_KERNEL=4 k_emit_mc4 for cubes.
 int emit_variant(const Plan& p, VolArgs* a) {
   const char* legacy = getenv("MSH_EMIT_V3");
   const bool lean = !(legacy && *legacy == '1');
+  const char* mck = getenv("MSH_MC_KERNEL");
+  if (lean && p.method == MC && !(mck && *mck == '4')) {
+    const int v = mc5_variant(p, a);
+    if (v >= 0) return 128 + v;
+  }
   if (lean && p.method == MC) {
     const int v = mc4_variant(p, a);
     if (v >= 0) return v;
@@ -907,6 +913,10 @@ static hipError_t emit_any(const Plan& p, int variant, const VolArgs& a, const V
   if (variant < 32) {
     if (vols) return launch_emit_mc4_batch(p, variant, vols, emap, gbase, nblocks, st);
     return launch_emit_mc4(p, variant, a, st);
+  }
+  if (variant >= 128) {
+    if (vols) return launch_emit_mc5_batch(p, variant - 128, vols, emap, gbase, nblocks, st);
+    return launch_emit_mc5(p, variant - 128, a, st);
   }
   if (variant >= 64) {
     if (vols) return launch_emit_halo_batch(p, variant - 64, vols, emap, gbase, nblocks, st);
