@@ -201,6 +201,15 @@ hipError_t launch_emit_mc5_batch(const Plan& p, int variant, const VolArgs* vols
                                  const int32_t* emap, int64_t gbase, int64_t nblocks,
                                  hipStream_t st);
 
+// halo5.hip: the Marching Tetrahedra and Naive Surface Nets emit kernel
+// with lanes along x.  -1: volume too large; else bit 0: dword DMA, bits
+// 1-2: axis types.  Fills the emit tiling.
+int h5_variant(const Plan& p, VolArgs* a);
+hipError_t launch_emit_h5(const Plan& p, int variant, const VolArgs& a, hipStream_t st);
+hipError_t launch_emit_h5_batch(const Plan& p, int variant, const VolArgs* vols,
+                                const int32_t* emap, int64_t gbase, int64_t nblocks,
+                                hipStream_t st);
+
 // tet.hip: the Marching Tetrahedra and Naive Surface Nets emit kernel.  -1:
 // volume too large; else bit 0: dword DMA, bits 1-2: axis types.  Fills the
 // emit tiling.

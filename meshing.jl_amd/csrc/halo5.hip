@@ -1,0 +1,576 @@
+// halo5.hip -- Marching Tetrahedra (src/marching_tetrahedra.jl:105-163) and
+// Naive Surface Nets (this library's definition, oracle/cpu_ref.cpp
+// sn_isosurface) emit kernels with lanes along x.
+//
+// The layout of the Marching Cubes kernel with lanes along x (mc5.hip): lane
+// i of wave w holds the cell row (x0 + i, jb - 1 + w) and marches it through
+// the z chunk with running in-row prefixes.  A face of these methods names
+// vertices owned by the cells at -x, -y and -z, so the block classifies one
+// more row and column than it emits: wave 0 holds the row at -y (it only
+// classifies), lane 3 the column at -x.  Every cell's case and in-row vertex
+// prefix go to LDS, a barrier makes them visible, and the emitting waves
+// (1..3) look the owners up there; the cell just below the chunk comes from
+// one extra plane of the brick.
+#include <stdlib.h>
+
+#include <type_traits>
+
+#include "device.h"
+
+#define MSH_LUT static __constant__
+#include "luts.h"
+
+namespace msh {
+namespace {
+
+constexpr int kH5TX = 56;  // cells a block emits along x: lanes 4..59
+constexpr int kH5CY = 3;   // cell rows a block emits along y: waves 1..3
+
+template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM>
+__device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
+  static_assert(METHOD == MT || METHOD == SN, "tetrahedra or surface nets");
+  static_assert(ZC <= 16, "z fits 4 bits of a queue entry");
+  constexpr int SX = 64, SY = 5, NP = ZC + 2;  // planes zc-1 .. zc+ZC, rows jb-1 .. jb+3
+  constexpr int DW = (int)(sizeof(T) / 4), RD = SX * DW;
+  constexpr int CAPE = 128, CAPV = 1024, CAPF = 1024;  // per wave
+  static_assert(kH5TX * 13 <= CAPV && kH5TX * 12 <= CAPF && kH5TX <= CAPE, "one z step must fit");
+  constexpr int K = METHOD == MT ? 1 : 0;
+  struct Smem {
+    uint32_t smp[2][NP * SY * RD];
+    double cz[2][NP];
+    double cx[SX], cy[SY];
+    // tetrahedra only (size 1 for surface nets)
+    uint8_t nown[K ? 8 * 256 : 1];    // vertices created, by class and (natural) case
+    uint8_t nf[K ? 256 : 1];          // faces, by case
+    uint16_t face[K ? 256 * 12 : 1];  // edges of each face, 5 bits each
+    uint8_t rank0[K ? 256 * 19 : 1];  // rank of each edge among a class-0 voxel's vertices
+    uint8_t own0[K ? 256 * 8 : 1];    // edges of a class-0 voxel's vertices, in order
+    uint8_t shift[K ? 152 : 1];       // MT_SHIFT
+    uint8_t cc[kWaves][K ? ZC : 1][64];  // case of each cell of the chunk
+    uint8_t pc[kWaves][K ? 64 : 1];      // case of the cell just below the chunk
+    uint32_t edge[19];                // per edge: corner offsets and axis bits
+    uint16_t cp[kWaves][ZC][64];      // in-row vertex prefix of each cell of the chunk
+    int64_t rb[kWaves][64];           // row vertex offset at the chunk start
+    int64_t dv[kWaves][64], df[kWaves][64];  // per row: output index minus queue slot
+    // queue entries: x | z << 6 | case << 10 | class << 18 | first slot << 21
+    uint32_t vlist[kWaves][CAPE], flist[kWaves][CAPE];
+    uint64_t vbits[kWaves][CAPV / 64], fbits[kWaves][CAPF / 64];
+  };
+  __shared__ Smem sm;
+  const T* __restrict__ f = (const T*)a.f;
+  Out* __restrict__ verts = (Out*)a.verts;
+  int64_t* __restrict__ faces = a.faces;
+  const Geom g = a.g;
+  const int lane = lane_id(), wv = wave_id();
+  const int ib = (int)(t % a.nti) * kH5TX;
+  const int64_t rest = t / a.nti;
+  const int jb = (int)(rest % a.ntj) * kH5CY;
+  const int s = (int)(rest / a.ntj);
+  const int zs = s * g.zseg, ze = min(zs + g.zseg, g.nz - 1);
+  const T thr = (T)a.thr;
+  const M iso = (M)a.iso;
+  const A eps = (A)a.eps, ome = (A)a.ome;
+  const int cxm = CXM >= 0 ? CXM : a.cxmask;
+  const bool cx32 = cxm & 1, cy32 = (cxm >> 1) & 1, cz32 = (cxm >> 2) & 1;
+  const double* zp = a.coords + g.nx + g.ny;
+  const int64_t vbase = a.sz[0], totv = a.sz[1], fbase = a.sz[2], totf = a.sz[3];
+  const int64_t pstride = (int64_t)g.nx * g.ny;
+  const int64_t nelem = pstride * g.nz;
+  const int x0 = ib - 4, y0 = jb - 1;  // brick origin: sample column 0, row 0
+
+  if constexpr (METHOD == MT) {
+    for (int c = threadIdx.x; c < 256; c += kBlock) {
+      const uint32_t m = nat_to_mt((uint32_t)c);
+#pragma unroll
+      for (int cls = 0; cls < 8; ++cls) sm.nown[cls * 256 + c] = MT_NOWN[cls * 256 + m];
+      const int nf = MT_NF[m];
+      sm.nf[c] = (uint8_t)nf;
+      for (int q = 0; q < 12; ++q) {
+        uint16_t w = 0;
+        if (q < nf)
+          w = (uint16_t)(MT_FACE[m * 36 + 3 * q] | (MT_FACE[m * 36 + 3 * q + 1] << 5) |
+                         (MT_FACE[m * 36 + 3 * q + 2] << 10));
+        sm.face[c * 12 + q] = w;
+      }
+      for (int e = 0; e < 19; ++e) sm.rank0[c * 19 + e] = MT_RANK[m * 19 + e];
+      for (int q = 0; q < 8; ++q) sm.own0[c * 8 + q] = q < 7 ? MT_OWN_EDGES[m * 20 + q] : 0;
+    }
+    for (int q = threadIdx.x; q < 152; q += kBlock) sm.shift[q] = MT_SHIFT[q];
+  }
+  if (threadIdx.x < (METHOD == MT ? 19 : 12)) {
+    // edge e from corner ca to cb (tetrahedron edges in MT corner order, or
+    // the 12 cube edges in MC corner order): brick offsets (dwords from the
+    // cell's corner 0) | offset of cb << 10 | (dx,dy,dz) of ca << 20 | of
+    // cb << 23 | ca << 26 | cb << 29 (corner ids, MC order, surface nets)
+    const uint32_t ew = METHOD == MT ? MT_EDGE[threadIdx.x] : MC_EDGE[threadIdx.x];
+    const int ca = ew & 0xF, cb = ew >> 4;
+    const int xa = METHOD == MT ? mt_dx(ca) : mc_dx(ca), ya = METHOD == MT ? mt_dy(ca) : mc_dy(ca),
+              za = METHOD == MT ? mt_dz(ca) : mc_dz(ca);
+    const int xb = METHOD == MT ? mt_dx(cb) : mc_dx(cb), yb = METHOD == MT ? mt_dy(cb) : mc_dy(cb),
+              zb = METHOD == MT ? mt_dz(cb) : mc_dz(cb);
+    const uint32_t oa = (uint32_t)((za * SY + ya) * RD + xa * DW);
+    const uint32_t ob = (uint32_t)((zb * SY + yb) * RD + xb * DW);
+    sm.edge[threadIdx.x] = oa | (ob << 10) | ((uint32_t)(xa | (ya << 1) | (za << 2)) << 20) |
+                           ((uint32_t)(xb | (yb << 1) | (zb << 2)) << 23) |
+                           (METHOD == SN ? ((uint32_t)ca << 26) | ((uint32_t)cb << 29) : 0u);
+  }
+  for (int q = threadIdx.x; q < SX; q += kBlock) sm.cx[q] = a.coords[min(max(x0 + q, 0), g.nx - 1)];
+  if (threadIdx.x < SY)
+    sm.cy[threadIdx.x] = a.coords[g.nx + min(max(y0 + (int)threadIdx.x, 0), g.ny - 1)];
+
+  // this lane's cell row
+  const int gi = x0 + lane, gj = y0 + wv;
+  const bool exists = lane < SX - 1 && gi >= 0 && gi < g.nx - 1 && gj >= 0 && gj < g.ny - 1;
+  const bool emits = exists && wv >= 1 && lane >= 4 && lane < 4 + kH5TX;
+  int64_t rbv = 0, rbf = 0;
+  if (exists) {
+    const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + gj) * g.nzseg + s;
+    rbv = a.voff[u] - vbase;
+    rbf = a.foff[u] - fbase;
+  }
+
+  // DMA: planes zc-1 .. zc+ZC, rows y0 .. y0+4, 64 samples from x0.  Near
+  // the volume's low faces the origin is outside the field: the buffer starts
+  // at the first sample that exists and lanes before it read 0.
+  constexpr int NROWS = NP * SY;
+  constexpr int RPI = VEC ? 1024 / (4 * RD) : 1;
+  constexpr int NI = VEC ? (NROWS + RPI - 1) / RPI : NROWS * DW;
+  constexpr int NIW = (NI + kWaves - 1) / kWaves;
+  uint32_t doff[NIW];
+  bool dok[NIW];
+#pragma unroll
+  for (int m = 0; m < NIW; ++m) {
+    const int ins = wv + m * kWaves;
+    int row, byte_in_row;
+    if (VEC) {
+      row = ins * RPI + lane / (kWave / RPI);
+      byte_in_row = (lane % (kWave / RPI)) * 16;
+    } else {
+      row = ins / DW;
+      byte_in_row = (ins % DW) * 256 + lane * 4;
+    }
+    dok[m] = ins < NI && row < NROWS;
+    const int pl = row / SY, y = row - pl * SY;
+    doff[m] = (uint32_t)(((int64_t)pl * pstride + (int64_t)y * g.nx) * (int64_t)sizeof(T)) +
+              (uint32_t)byte_in_row;
+  }
+  auto fetch = [&](int zc, int buf) {
+    const int64_t start = (int64_t)(zc - 1) * pstride + (int64_t)y0 * g.nx + x0;
+    const int64_t first = start < 0 ? 0 : start;
+    const uint32_t skip = (uint32_t)((first - start) * (int64_t)sizeof(T));
+    const int64_t left = (nelem - first) * (int64_t)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs =
+        buffer_rsrc(f + first, (uint32_t)(left < 0x7FFFFFFF ? left : 0x7FFFFFFF));
+#pragma unroll
+    for (int m = 0; m < NIW; ++m) {
+      const int ins = wv + m * kWaves;
+      // 0x80000000 is past every buffer's range: those lanes read zeros
+      const uint32_t off = doff[m] >= skip ? doff[m] - skip : 0x80000000u;
+      if (dok[m]) dma_to_lds<VEC ? 16 : 4>(rs, sm.smp[buf] + ins * (VEC ? 256 : kWave), off);
+    }
+    constexpr int NZ = (2 * NP + kWave - 1) / kWave;
+    uint32_t* zd = (uint32_t*)sm.cz[buf];
+    for (int ins = wv; ins < NZ; ins += kWaves) {
+      const int p = ins * kWave + lane;
+      if (p < 2 * NP) {
+        const double* src = zp + min(max(zc - 1 + (p >> 1), 0), g.nz - 1);
+        __builtin_amdgcn_global_load_lds((const void*)((const char*)src + 4 * (p & 1)),
+                                         (void*)(zd + ins * kWave), 4, 0, 0);
+      }
+    }
+  };
+
+  uint32_t* vlist = sm.vlist[wv];
+  uint32_t* flist = sm.flist[wv];
+  uint64_t* vbits = sm.vbits[wv];
+  uint64_t* fbits = sm.fbits[wv];
+  for (int q = lane; q < CAPV / 64; q += kWave) vbits[q] = 0;
+  for (int q = lane; q < CAPF / 64; q += kWave) fbits[q] = 0;
+
+  if (zs < ze) fetch(zs, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  lds_barrier();
+  int buf = 0;
+  for (int zc = zs; zc < ze; zc += ZC, buf ^= 1) {
+    if (zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
+    const uint32_t* smp = sm.smp[buf];
+    const double* czs = sm.cz[buf];
+
+    // classify this lane's row: natural-order face nibble per plane
+    uint32_t nib[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const T v0 = *(const T*)(smp + (p * SY + wv) * RD + lane * DW);
+      const T v1 = *(const T*)(smp + (p * SY + wv + 1) * RD + lane * DW);
+      const uint32_t w = (uint32_t)(v0 < thr) | ((uint32_t)(v1 < thr) << 2);
+      nib[p] = w | (next_lane(w, 0u) << 1);
+    }
+    const int zn = min(ZC, ze - zc);
+    uint32_t cs[ZC], nvz[ZC], nfz[ZC];
+    uint32_t runv = 0, tot = 0;  // in-row vertex prefix; emitted vertices | faces << 16
+#pragma unroll
+    for (int z = 0; z < ZC; ++z) {
+      const int gk = zc + z;
+      cs[z] = (exists && z < zn) ? (nib[z + 1] | (nib[z + 2] << 4)) : 0u;
+      const uint32_t cls = (uint32_t)((gi == 0) | ((gj == 0) << 1) | ((gk == 0) << 2));
+      if constexpr (METHOD == MT) {
+        nvz[z] = sm.nown[cls * 256 + cs[z]];
+        nfz[z] = emits ? (uint32_t)sm.nf[cs[z]] : 0u;
+        sm.cc[wv][z][lane] = (uint8_t)cs[z];
+      } else {
+        const uint32_t csm = nat_to_mc(cs[z]), c0 = csm & 1;
+        nvz[z] = (cs[z] != 0 && cs[z] != 255) ? 1u : 0u;
+        const uint32_t nq = (uint32_t)((c0 != ((csm >> 1) & 1)) && gj >= 1 && gk >= 1) +
+                            (uint32_t)((c0 != ((csm >> 3) & 1)) && gk >= 1 && gi >= 1) +
+                            (uint32_t)((c0 != ((csm >> 4) & 1)) && gi >= 1 && gj >= 1);
+        nfz[z] = (emits && nvz[z]) ? 2u * nq : 0u;
+      }
+      sm.cp[wv][z][lane] = (uint16_t)runv;
+      runv += nvz[z];
+      if (emits) tot += nvz[z] | (nfz[z] << 16);
+    }
+    if constexpr (METHOD == MT) sm.pc[wv][lane] = (uint8_t)((exists && zc > 0) ? (nib[0] | (nib[1] << 4)) : 0u);
+    sm.rb[wv][lane] = rbv;
+    // this wave's share of the next brick must land before the barrier that
+    // ends the chunk; wait now, before the queue's stores are issued
+    __builtin_amdgcn_s_waitcnt(0);
+    lds_barrier();  // every row's cases and prefixes are in LDS
+
+    if (wv >= 1) {
+      int nvl = 0, nfl = 0;
+      uint32_t V = 0, F = 0;
+      auto flush = [&]() {
+        int before = 0;
+        for (uint32_t o0 = 0; o0 < V; o0 += kWave) {
+          const uint32_t o = o0 + lane;
+          const uint64_t word = vbits[o0 >> 6];
+          const uint64_t upto = word & (~0ull >> (kWave - 1 - lane));
+          const int k = before + __popcll(upto) - 1;
+          before += __popcll(word);
+          if (o >= V) break;
+          const uint32_t e = vlist[k];
+          const int lc = (int)(e & 63), c = (int)((e >> 6) & 15);
+          const uint32_t ecs = (e >> 10) & 0xFF, cls = (e >> 18) & 7, sv = e >> 21;
+          const uint32_t q = o - sv;
+          const uint32_t* cell = smp + ((c + 1) * SY + wv) * RD + lc * DW;
+          double px, py, pz;
+          if constexpr (METHOD == MT) {
+            // class-0 voxels (all but the first layers) read the LDS copy
+            uint32_t ed = sm.own0[ecs * 8 + (q & 7)];
+            if (cls != 0) ed = MT_OWN_EDGES[(cls * 256 + nat_to_mt(ecs)) * 20 + q];
+            const uint32_t ew = sm.edge[ed];
+            const T sv0 = *(const T*)(cell + (ew & 0x3FF)), tv = *(const T*)(cell + ((ew >> 10) & 0x3FF));
+            // vertPos (src/marching_tetrahedra.jl:42-55)
+            const M rr = (iso - (M)sv0) / (M)(tv - sv0);
+            const A aw = jl_min<A>(jl_max<A>((A)rr, eps), ome);
+            const A bw = (A)1 - aw;
+            const A wx = (A)((ew >> 20) & 1) * bw + (A)((ew >> 23) & 1) * aw;
+            const A wy = (A)((ew >> 21) & 1) * bw + (A)((ew >> 24) & 1) * aw;
+            const A wz = (A)((ew >> 22) & 1) * bw + (A)((ew >> 25) & 1) * aw;
+            px = mt_axis<A>(cx32, wx, sm.cx[lc], sm.cx[lc + 1]);
+            py = mt_axis<A>(cy32, wy, sm.cy[wv], sm.cy[wv + 1]);
+            pz = mt_axis<A>(cz32, wz, czs[c + 1], czs[c + 2]);
+          } else {
+            // the mean of the crossing points of the cell's 12 edges, each
+            // interpolated as vertex_interp does (src/marching_cubes.jl:100-104),
+            // summed per axis in promote(M, axis type)
+            const uint32_t csm = nat_to_mc(ecs);
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            double dx = 0.0, dy = 0.0, dz = 0.0;
+            const bool sx = sizeof(M) == 4 && cx32, sy = sizeof(M) == 4 && cy32,
+                       sz = sizeof(M) == 4 && cz32;
+            int ncross = 0;
+            for (int ed = 0; ed < 12; ++ed) {
+              const uint32_t ew = sm.edge[ed];
+              if (((csm >> ((ew >> 26) & 7)) & 1) == ((csm >> (ew >> 29)) & 1)) continue;
+              ++ncross;
+              const T va = *(const T*)(cell + (ew & 0x3FF)), vb = *(const T*)(cell + ((ew >> 10) & 0x3FF));
+              const M mu = (iso - (M)va) / (M)(vb - va);
+              const double qx = lerp_axis<M>(cx32, mu, sm.cx[lc + ((ew >> 20) & 1)], sm.cx[lc + ((ew >> 23) & 1)]);
+              const double qy = lerp_axis<M>(cy32, mu, sm.cy[wv + ((ew >> 21) & 1)], sm.cy[wv + ((ew >> 24) & 1)]);
+              const double qz = lerp_axis<M>(cz32, mu, czs[c + 1 + ((ew >> 22) & 1)], czs[c + 1 + ((ew >> 25) & 1)]);
+              if (sx) fx = fx + (float)qx; else dx = dx + qx;
+              if (sy) fy = fy + (float)qy; else dy = dy + qy;
+              if (sz) fz = fz + (float)qz; else dz = dz + qz;
+            }
+            px = sx ? (double)(fx / (float)ncross) : dx / (double)ncross;
+            py = sy ? (double)(fy / (float)ncross) : dy / (double)ncross;
+            pz = sz ? (double)(fz / (float)ncross) : dz / (double)ncross;
+          }
+          const int64_t vi = sm.dv[wv][lc] + o;
+          if (vi < totv) {
+            Out* out = verts + vi * 3;
+            out[0] = (Out)px;
+            out[1] = (Out)py;
+            out[2] = (Out)pz;
+          }
+        }
+        before = 0;
+        for (uint32_t o0 = 0; o0 < F; o0 += kWave) {
+          const uint32_t o = o0 + lane;
+          const uint64_t word = fbits[o0 >> 6];
+          const uint64_t upto = word & (~0ull >> (kWave - 1 - lane));
+          const int k = before + __popcll(upto) - 1;
+          before += __popcll(word);
+          if (o >= F) break;
+          const uint32_t e = flist[k];
+          const int lc = (int)(e & 63), c = (int)((e >> 6) & 15);
+          const uint32_t ecs = (e >> 10) & 0xFF, sf = e >> 21;
+          const uint32_t q = o - sf;
+          const int ci = x0 + lc, cj = gj, ck = zc + c;
+          int64_t id[3];
+          if constexpr (METHOD == MT) {
+            const uint32_t fw = sm.face[ecs * 12 + q];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+              const uint32_t ed = (fw >> (5 * m)) & 31;
+              const uint32_t ew = sm.edge[ed];
+              // the owner is one cell back along each axis on which the edge's
+              // end corner has coordinate 0 (when that cell exists)
+              const int dx = (((ew >> 23) & 1) == 0) & (ci > 0);
+              const int dy = (((ew >> 24) & 1) == 0) & (cj > 0);
+              const int dz = (((ew >> 25) & 1) == 0) & (ck > 0);
+              const uint32_t e2 = sm.shift[ed * 8 + (dx | (dy << 1) | (dz << 2))];
+              const int l2 = lc - dx, w2 = wv - dy, z2 = c - dz;
+              const uint32_t cls2 =
+                  (uint32_t)((ci - dx == 0) | ((cj - dy == 0) << 1) | ((ck - dz == 0) << 2));
+              uint32_t oc;
+              int64_t off;
+              if (z2 >= 0) {
+                oc = sm.cc[w2][z2][l2];
+                off = sm.rb[w2][l2] + sm.cp[w2][z2][l2];
+              } else {
+                oc = sm.pc[w2][l2];
+                off = sm.rb[w2][l2] - (int64_t)sm.nown[cls2 * 256 + oc];
+              }
+              uint32_t rk = sm.rank0[oc * 19 + e2];
+              if (cls2 != 0) rk = MT_RANK[(cls2 * 256 + nat_to_mt(oc)) * 19 + e2];
+              id[m] = off + (int64_t)rk + 1;
+            }
+          } else {
+            // quads around the crossing edges at corner 0, two triangles each,
+            // axes in x, y, z order; the q-th triangle is triangle q&1 of the
+            // (q>>1)-th quad
+            const uint32_t csm = nat_to_mc(ecs), c0 = csm & 1;
+            const uint32_t axes = (uint32_t)((c0 != ((csm >> 1) & 1)) && cj >= 1 && ck >= 1) |
+                                  ((uint32_t)((c0 != ((csm >> 3) & 1)) && ck >= 1 && ci >= 1) << 1) |
+                                  ((uint32_t)((c0 != ((csm >> 4) & 1)) && ci >= 1 && cj >= 1) << 2);
+            const uint32_t first = axes & (0u - axes);
+            const uint32_t pick = (q >> 1) == 0 ? first
+                                : (q >> 1) == 1 ? ((axes ^ first) & (0u - (axes ^ first))) : 4u;
+            const int ax = pick == 1 ? 0 : pick == 2 ? 1 : 2;
+            const int u = (ax + 1) % 3, w = (ax + 2) % 3;
+            auto cell_id = [&](int du, int dw) -> int64_t {
+              const int d0 = (u == 0 ? du : 0) + (w == 0 ? dw : 0);
+              const int d1 = (u == 1 ? du : 0) + (w == 1 ? dw : 0);
+              const int d2 = (u == 2 ? du : 0) + (w == 2 ? dw : 0);
+              const int l2 = lc - d0, w2 = wv - d1, z2 = c - d2;
+              if (z2 >= 0) return sm.rb[w2][l2] + sm.cp[w2][z2][l2] + 1;
+              return sm.rb[w2][l2];  // the cell just below the chunk, active: the row's last vertex
+            };
+            const int64_t me = sm.rb[wv][lc] + sm.cp[wv][c][lc] + 1;
+            const int64_t q1 = cell_id(1, 0), q2 = cell_id(1, 1), q3 = cell_id(0, 1);
+            const int64_t a1 = c0 ? q1 : q3, a3 = c0 ? q3 : q1;
+            id[0] = me;
+            id[1] = (q & 1) ? q2 : a1;
+            id[2] = (q & 1) ? a3 : q2;
+          }
+          const int64_t fi = sm.df[wv][lc] + o;
+          if (fi < totf) {
+            int64_t* out = faces + fi * 3;
+            out[0] = id[0];
+            out[1] = id[1];
+            out[2] = id[2];
+          }
+        }
+        wave_sync();
+        for (uint32_t q = lane; q < (V + 63) / 64; q += kWave) vbits[q] = 0;
+        for (uint32_t q = lane; q < (F + 63) / 64; q += kWave) fbits[q] = 0;
+      };
+
+      // queue the emitting cells z in [za, zb) of every row of this wave,
+      // rows one after another, each row's cells in z order
+      auto append = [&](int za, int zb, uint32_t rtot, uint32_t run0) {
+        uint32_t nev = 0, nef = 0;
+#pragma unroll
+        for (int z = 0; z < ZC; ++z) {
+          const bool in = z >= za && z < zb && emits;
+          nev += (in && nvz[z] != 0) ? 1u : 0u;
+          nef += (in && nfz[z] != 0) ? 1u : 0u;
+        }
+        const uint32_t inct = wave_scan_dpp(rtot), incv = wave_scan_dpp(nev), incf = wave_scan_dpp(nef);
+        const uint32_t sumt = (uint32_t)__builtin_amdgcn_readlane((int)inct, kWave - 1);
+        const uint32_t sumv = (uint32_t)__builtin_amdgcn_readlane((int)incv, kWave - 1);
+        const uint32_t sumf = (uint32_t)__builtin_amdgcn_readlane((int)incf, kWave - 1);
+        const uint32_t bv = V + ((inct - rtot) & 0xFFFF), bf = F + ((inct - rtot) >> 16);
+        uint32_t kv = (uint32_t)nvl + incv - nev, kf = (uint32_t)nfl + incf - nef;
+        if (nev | nef) {
+          sm.dv[wv][lane] = rbv + (int64_t)(run0 & 0xFFFF) - (int64_t)bv;
+          sm.df[wv][lane] = rbf + (int64_t)(run0 >> 16) - (int64_t)bf;
+        }
+        uint32_t run = 0;
+#pragma unroll
+        for (int z = 0; z < ZC; ++z) {
+          const bool in = z >= za && z < zb && emits;
+          if (in) {
+            const uint32_t cls = (uint32_t)((gi == 0) | ((gj == 0) << 1) | ((zc + z == 0) << 2));
+            const uint32_t head = (uint32_t)lane | ((uint32_t)z << 6) | (cs[z] << 10) | (cls << 18);
+            if (nvz[z] != 0) {
+              const uint32_t sv = bv + (run & 0xFFFF);
+              vlist[kv++] = head | (sv << 21);
+              atomicOr((unsigned long long*)&vbits[sv >> 6], 1ull << (sv & 63));
+            }
+            if (nfz[z] != 0) {
+              const uint32_t sf = bf + (run >> 16);
+              flist[kf++] = head | (sf << 21);
+              atomicOr((unsigned long long*)&fbits[sf >> 6], 1ull << (sf & 63));
+            }
+            run += nvz[z] | (nfz[z] << 16);
+          }
+        }
+        nvl += (int)sumv;
+        nfl += (int)sumf;
+        V += sumt & 0xFFFF;
+        F += sumt >> 16;
+      };
+
+      const uint32_t ttot = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_dpp(tot), kWave - 1);
+      uint32_t cv = 0, cf = 0;
+#pragma unroll
+      for (int z = 0; z < ZC; ++z) {
+        cv += (emits && nvz[z] != 0) ? 1u : 0u;
+        cf += (emits && nfz[z] != 0) ? 1u : 0u;
+      }
+      const uint32_t tcv = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_dpp(cv), kWave - 1);
+      const uint32_t tcf = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_dpp(cf), kWave - 1);
+      // the whole chunk in one go when it fits the queue, else one z step at a time
+      const bool whole = (ttot & 0xFFFF) <= (uint32_t)CAPV && (ttot >> 16) <= (uint32_t)CAPF &&
+                         tcv <= (uint32_t)CAPE && tcf <= (uint32_t)CAPE;
+      uint32_t run0 = 0;
+      for (int za = 0; ttot != 0 && za < zn;) {
+        const int zb = whole ? ZC : za + 1;
+        uint32_t part = 0;
+#pragma unroll
+        for (int q = 0; q < ZC; ++q)
+          part += (emits && q >= za && q < zb) ? (nvz[q] | (nfz[q] << 16)) : 0u;
+        append(za, zb, part, run0);
+        run0 += part;
+        wave_sync();
+        flush();
+        wave_sync();
+        nvl = 0;
+        nfl = 0;
+        V = 0;
+        F = 0;
+        za = zb;
+      }
+    }
+    rbv += runv;
+    rbf += tot >> 16;
+    lds_barrier();
+  }
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM>
+__global__ __launch_bounds__(kBlock) void k_emit_h5(VolArgs a) {
+  emit_h5_tile<METHOD, T, M, A, Out, ZC, VEC, CXM>(a, xcd_tile(blockIdx.x, a.ntiles));
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM>
+__global__ __launch_bounds__(kBlock) void k_emit_h5_b(const VolArgs* __restrict__ vols,
+                                                      const int32_t* __restrict__ emap,
+                                                      int64_t gbase) {
+  const int64_t b = gbase + blockIdx.x;
+  const VolArgs a = vols[emap[b]];
+  emit_h5_tile<METHOD, T, M, A, Out, ZC, VEC, CXM>(a, b - a.eblk0);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+template <typename T>
+static constexpr int h5_zc() { return sizeof(T) == 4 ? 8 : 4; }
+
+int h5_variant(const Plan& p, VolArgs* a) {
+  const int zc = p.T == K_F32 ? h5_zc<float>() : h5_zc<double>();
+  const int64_t esz = p.T == K_F32 ? 4 : 8;
+  // the brick's 32-bit DMA offsets reach plane zc+1, row 4, sample 63
+  if (((int64_t)(zc + 1) * a->g.nx * a->g.ny + 4ll * a->g.nx + 64) * esz >= (1ll << 31)) return -1;
+  a->nti = (a->g.nx - 1 + kH5TX - 1) / kH5TX;
+  a->ntj = (a->g.ny - 1 + kH5CY - 1) / kH5CY;
+  a->ntiles = (a->g.nx > 1 && a->g.ny > 1) ? (int64_t)a->nti * a->ntj * a->g.nzseg : 0;
+  // 16-byte DMA: rows 16-byte aligned (the brick starts 4 cells before a
+  // tile whose x is a multiple of 56, so that start is aligned too)
+  const bool vec = ((uintptr_t)a->f % 16 == 0) && (((int64_t)a->g.nx * esz) % 16 == 0);
+  const int cxclass = a->cxmask == 0 ? 1 : a->cxmask == 7 ? 2 : 0;
+  return (vec ? 0 : 1) | (cxclass << 1);
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out, int CXM>
+static hipError_t h5_z(bool vec, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
+                       int64_t gbase, int64_t nblocks, hipStream_t st) {
+  constexpr int ZC = h5_zc<T>();
+  if (vols) {
+    if (vec)
+      hipLaunchKernelGGL((k_emit_h5_b<METHOD, T, M, A, Out, ZC, true, CXM>), dim3((unsigned)nblocks),
+                         dim3(kBlock), 0, st, vols, emap, gbase);
+    else
+      hipLaunchKernelGGL((k_emit_h5_b<METHOD, T, M, A, Out, ZC, false, CXM>), dim3((unsigned)nblocks),
+                         dim3(kBlock), 0, st, vols, emap, gbase);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL((k_emit_h5<METHOD, T, M, A, Out, ZC, true, CXM>), dim3((unsigned)a.ntiles),
+                         dim3(kBlock), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_emit_h5<METHOD, T, M, A, Out, ZC, false, CXM>), dim3((unsigned)a.ntiles),
+                         dim3(kBlock), 0, st, a);
+  }
+  return hipGetLastError();
+}
+
+template <int METHOD, typename T, typename M, typename A, typename Out>
+static hipError_t h5_t(int variant, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
+                       int64_t gbase, int64_t nblocks, hipStream_t st) {
+  const bool vec = (variant & 1) == 0;
+  switch ((variant >> 1) & 3) {
+    case 1: return h5_z<METHOD, T, M, A, Out, 0>(vec, a, vols, emap, gbase, nblocks, st);
+    case 2: return h5_z<METHOD, T, M, A, Out, 7>(vec, a, vols, emap, gbase, nblocks, st);
+    default: return h5_z<METHOD, T, M, A, Out, -1>(vec, a, vols, emap, gbase, nblocks, st);
+  }
+}
+
+static hipError_t h5_any(const Plan& p, int variant, const VolArgs& a, const VolArgs* vols,
+                         const int32_t* emap, int64_t gbase, int64_t nblocks, hipStream_t st) {
+#define MSH_ARGS variant, a, vols, emap, gbase, nblocks, st
+  if (p.method == SN) {
+    // no eps: A is M
+    if (p.T == K_F32 && p.M == K_F32 && p.Out == K_F32)
+      return h5_t<SN, float, float, float, float>(MSH_ARGS);
+    if (p.T == K_F32 && p.M == K_F32) return h5_t<SN, float, float, float, double>(MSH_ARGS);
+    if (p.T == K_F32) return h5_t<SN, float, double, double, double>(MSH_ARGS);
+    return h5_t<SN, double, double, double, double>(MSH_ARGS);
+  }
+  if (p.T == K_F32 && p.M == K_F32 && p.A == K_F32 && p.Out == K_F32)
+    return h5_t<MT, float, float, float, float>(MSH_ARGS);
+  if (p.T == K_F32 && p.M == K_F32 && p.A == K_F32) return h5_t<MT, float, float, float, double>(MSH_ARGS);
+  if (p.T == K_F32 && p.M == K_F32) return h5_t<MT, float, float, double, double>(MSH_ARGS);
+  if (p.T == K_F32) return h5_t<MT, float, double, double, double>(MSH_ARGS);
+  return h5_t<MT, double, double, double, double>(MSH_ARGS);
+#undef MSH_ARGS
+}
+
+hipError_t launch_emit_h5(const Plan& p, int variant, const VolArgs& a, hipStream_t st) {
+  if (a.ntiles == 0) return hipSuccess;
+  return h5_any(p, variant, a, nullptr, nullptr, 0, 0, st);
+}
+
+hipError_t launch_emit_h5_batch(const Plan& p, int variant, const VolArgs* vols,
+                                const int32_t* emap, int64_t gbase, int64_t nblocks,
+                                hipStream_t st) {
+  if (nblocks == 0) return hipSuccess;
+  return h5_any(p, variant, VolArgs{}, vols, emap, gbase, nblocks, st);
+}
+
+}  // namespace msh

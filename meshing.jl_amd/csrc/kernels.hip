@@ -808,8 +808,10 @@ static const int kShapeTX[4] = {31, 31, 15, 15}, kShapeTY[4] = {2, 1, 2, 1};
 
 // Variants: 0..31 = k_emit_mc4 (see mc4_variant); 32 + shape = k_emit;
 // 64 + v = k_emit_halo, tetrahedra and surface nets (see halo_variant);
-// 128 + v = k_emit_mc5 (see mc5_variant).  Comparison runs: MSH_EMIT_V3=1
-// selects k_emit for every method, MSH_MC_KERNEL=4 k_emit_mc4 for cubes.
+// 128 + v = k_emit_mc5 (see mc5_variant); 192 + v = k_emit_h5, tetrahedra
+// and surface nets with lanes along x (see h5_variant).  Comparison runs:
+// MSH_EMIT_V3=1 selects k_emit for every method, MSH_MC_KERNEL=4 k_emit_mc4
+// for cubes, MSH_HALO_KERNEL=4 k_emit_halo for tetrahedra and surface nets.
 int emit_variant(const Plan& p, VolArgs* a) {
   const char* legacy = getenv("MSH_EMIT_V3");
   const bool lean = !(legacy && *legacy == '1');
@@ -821,6 +823,11 @@ int emit_variant(const Plan& p, VolArgs* a) {
   if (lean && p.method == MC) {
     const int v = mc4_variant(p, a);
     if (v >= 0) return v;
+  }
+  const char* hk = getenv("MSH_HALO_KERNEL");
+  if (lean && (p.method == MT || p.method == SN) && !(hk && *hk == '4')) {
+    const int v = h5_variant(p, a);
+    if (v >= 0) return 192 + v;
   }
   if (lean && (p.method == MT || p.method == SN)) {
     const int v = halo_variant(p, a);
@@ -913,6 +920,10 @@ static hipError_t emit_any(const Plan& p, int variant, const VolArgs& a, const V
   if (variant < 32) {
     if (vols) return launch_emit_mc4_batch(p, variant, vols, emap, gbase, nblocks, st);
     return launch_emit_mc4(p, variant, a, st);
+  }
+  if (variant >= 192) {
+    if (vols) return launch_emit_h5_batch(p, variant - 192, vols, emap, gbase, nblocks, st);
+    return launch_emit_h5(p, variant - 192, a, st);
   }
   if (variant >= 128) {
     if (vols) return launch_emit_mc5_batch(p, variant - 128, vols, emap, gbase, nblocks, st);
