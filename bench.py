@@ -105,6 +105,7 @@ def measure(m, dtype, n, method_name, steps, warmup, kind="value_noise"):
             "sn": m.NaiveSurfaceNets(iso=0)}[method_name]
     field = m.generate_field(kind, (n, n, n), dtype, seed=1)
     torch.cuda.synchronize()
+    family = m.autotune(field, meth)  # the emit kernel family for this (method, dtype)
     r = Runner(m, field, meth)
     for _ in range(warmup):
         r.step()
@@ -135,6 +136,7 @@ def measure(m, dtype, n, method_name, steps, warmup, kind="value_noise"):
     field_bytes = N * esz
     out_bytes = r.nv * 3 * osz + r.nf * 3 * 8
     res = dict(dtype=dtype, samples=N, nverts=r.nv, nfaces=r.nf, step_s=dt, count_s=t_count,
+               family=m.KERNEL_FAMILIES[family],
                emit_s=t_emit, field_bytes=field_bytes, out_bytes=out_bytes,
                gvox_s=N / dt / 1e9, tris_s=r.nf / dt)
     del r, field
@@ -409,6 +411,7 @@ def main():
     for meth in ("mt", "sn", "mc"):
         r = measure(m, "float32", 256, meth, a.steps, a.warmup, kind="gyroid")
         gyroid[meth] = dict(gvoxels_s=round(r["gvox_s"], 3), tris_s=round(r["tris_s"], 1),
+                            autotuned_emit_kernels=r["family"],
                             ms_per_step=round(r["step_s"] * 1e3, 4),
                             count_ms=round(r["count_s"] * 1e3, 4),
                             emit_ms=round(r["emit_s"] * 1e3, 4),
@@ -435,6 +438,7 @@ def main():
                     ms_per_step=round(r["step_s"] * 1e3, 4),
                     count_ms=round(r["count_s"] * 1e3, 4), emit_ms=round(r["emit_s"] * 1e3, 4),
                     nverts=r["nverts"], nfaces=r["nfaces"], algorithmic_bytes=b,
+                    autotuned_emit_kernels=r["family"],
                     step_hbm_frac=round(b / r["step_s"] / HBM_PEAK, 4),
                     roofline=roofline(r))
 

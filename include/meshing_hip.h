@@ -110,6 +110,19 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
 int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
                       void* const* verts, int64_t* const* faces, int32_t out_memory);
 
+/* Kernel choice.  The emit pass has three kernel families per method: 2
+ * lanes along x (the default), 1 lanes along z, 0 a generic kernel; each
+ * family falls back to the next one down for volumes it cannot take.
+ * msh_autotune counts the given field, times the emit of every family that
+ * takes it (three launches each on the context's stream, into a scratch
+ * buffer) and records the fastest for the field's (method, element type) in
+ * the context, where later msh_count and msh_batch_count calls use it;
+ * *family returns the choice.  msh_set_kernel_family sets it directly (-1
+ * restores the default).  Results do not depend on the family.  No
+ * counterpart in the reference (its tile shapes are the CPU loop). */
+int msh_autotune(void* ctx, const msh_field* field, const msh_params* params, int32_t* family);
+int msh_set_kernel_family(void* ctx, int32_t method, int32_t dtype, int32_t family);
+
 /* Synthetic fields, generated on the device into a column-major buffer of
  * nx*ny*nz elements of `dtype`.  kind: 0 sphere sqrt(x^2+y^2+z^2)-p0 on
  * [-1,1]^3; 1 torus (R=p0, r=p1) on [-1,1]^3; 2 gyroid on [-p0,p0]^3;

@@ -34,7 +34,10 @@ from ._capi import (MSH_DEVICE, MSH_FLOAT32, MSH_FLOAT64, MSH_HOST, MSH_INT64, M
 
 __all__ = ["isosurface", "isosurface_batch", "AbstractMeshingAlgorithm", "MarchingCubes",
            "MarchingTetrahedra", "NaiveSurfaceNets", "device_field", "generate_field", "count",
-           "Server", "FieldSpec"]
+           "autotune", "set_kernel_family", "KERNEL_FAMILIES", "Server", "FieldSpec"]
+
+# emit kernel families, best first (see msh_autotune in include/meshing_hip.h)
+KERNEL_FAMILIES = {2: "lanes along x", 1: "lanes along z", 0: "generic", -1: "default"}
 
 
 class AbstractMeshingAlgorithm:
@@ -330,6 +333,37 @@ def isosurface_batch(fields, methods=None, axes=None, device=0):
             mem = MSH_HOST
         ctx.check(L.msh_batch_extract(ctx.handle, n, F_arr, P_arr, vp, fp, mem))
     return out
+
+
+def autotune(sdf, method=None, *args, X=(-1, 1), Y=(-1, 1), Z=(-1, 1), device=0):
+    """Time every emit kernel family on this field and method and keep the
+    fastest for its (method, element type) in the device's context; later
+    calls of isosurface, isosurface_batch and the server on that device use
+    it.  Returns the chosen family (2 lanes along x, 1 lanes along z, 0
+    generic; see KERNEL_FAMILIES)."""
+    method, X, Y, Z = _split_args(method, args, X, Y, Z)
+    f, keep, on_dev = _field(sdf)
+    p = _params(method, X, Y, Z)
+    ctx = context(keep.device.index if on_dev else device)
+    fam = ctypes.c_int32()
+    with ctx.lock:
+        if on_dev:
+            ctx.check(_capi.lib().msh_set_stream(
+                ctx.handle, ctypes.c_void_p(_torch().cuda.current_stream(keep.device).cuda_stream)))
+        ctx.check(_capi.lib().msh_autotune(ctx.handle, ctypes.byref(f), ctypes.byref(p),
+                                           ctypes.byref(fam)))
+    return fam.value
+
+
+def set_kernel_family(method, dtype, family, device=0):
+    """Choose the emit kernel family for (method, element type) on a device
+    (-1: default order).  Results do not depend on it."""
+    m = method.method if isinstance(method, AbstractMeshingAlgorithm) else int(method)
+    name = getattr(dtype, "__name__", None) or str(dtype).replace("torch.", "")
+    kd = MSH_FLOAT32 if name == "float32" else MSH_FLOAT64
+    ctx = context(device)
+    with ctx.lock:
+        ctx.check(_capi.lib().msh_set_kernel_family(ctx.handle, m, kd, int(family)))
 
 
 FIELD_KINDS = {"sphere": 0, "torus": 1, "gyroid": 2, "value_noise": 3, "blobs": 4}

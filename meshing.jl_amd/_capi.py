@@ -19,7 +19,8 @@ MSH_HOST, MSH_DEVICE = range(2)
 # every symbol the header declares; checked by the CPU test suite
 EXPORTS = ("msh_create", "msh_destroy", "msh_set_stream", "msh_synchronize",
            "msh_last_error", "msh_output_kind", "msh_count", "msh_extract",
-           "msh_batch_count", "msh_batch_extract", "msh_generate_field")
+           "msh_batch_count", "msh_batch_extract", "msh_autotune", "msh_set_kernel_family",
+           "msh_generate_field")
 
 
 class MshField(ctypes.Structure):
@@ -82,6 +83,8 @@ def lib():
                                           ctypes.POINTER(i64)]
             L.msh_batch_extract.argtypes = [vp, i32, pf, pp, ctypes.POINTER(vp),
                                             ctypes.POINTER(vp), i32]
+            L.msh_autotune.argtypes = [vp, pf, pp, ctypes.POINTER(i32)]
+            L.msh_set_kernel_family.argtypes = [vp, i32, i32, i32]
             L.msh_generate_field.argtypes = [vp, i32, i32, i64, i64, i64, ctypes.c_uint64,
                                              ctypes.c_double, ctypes.c_double, vp]
             for name in EXPORTS:

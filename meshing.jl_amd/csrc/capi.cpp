@@ -63,6 +63,9 @@ struct Ctx {
   hipEvent_t upload = nullptr;  // last descriptor upload from hbatch
   bool upload_pending = false;
   int64_t* pinned = nullptr;  // 4 x int64 for the totals
+  // emit kernel family per (method, element type), -1 = default order; set
+  // by msh_autotune or msh_set_kernel_family
+  int family[3][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};
   // state of the last successful msh_count
   bool have = false;
   msh_field field{};
@@ -386,6 +389,7 @@ int msh_count(void* ctx, const msh_field* f, const msh_params* p, int64_t* nvert
   Plan pl;
   int rc = make_plan(c, f, p, &pl);
   if (rc) return rc;
+  pl.family = c->family[pl.method][pl.T];
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
   Geom g = make_geom(f);
   Workspace w{};
@@ -490,6 +494,7 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
       c->err = "volume " + std::to_string(v) + ": " + c->err;
       return rc;
     }
+    b.plans[v].family = c->family[b.plans[v].method][b.plans[v].T];
     geoms[v] = make_geom(&fields[v], true);
     if (geoms[v].units > 0 && !fields[v].data)
       return fail(c, MSH_ERR_ARGUMENT, "volume " + std::to_string(v) + ": null field data");
@@ -705,6 +710,69 @@ int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_p
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "output download");
   }
+  return MSH_OK;
+}
+
+int msh_set_kernel_family(void* ctx, int32_t method, int32_t dtype, int32_t family) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  if (method < 0 || method > 2 || (dtype != K_F32 && dtype != K_F64) || family < -1 ||
+      family > 2)
+    return fail(c, MSH_ERR_ARGUMENT, "method, dtype or kernel family out of range");
+  c->family[method][dtype] = family;
+  c->have = false;
+  c->batch.have = false;
+  return MSH_OK;
+}
+
+int msh_autotune(void* ctx, const msh_field* f, const msh_params* p, int32_t* family) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  int64_t nv = 0, nf = 0;
+  int rc = msh_count(ctx, f, p, &nv, &nf);
+  if (rc) return rc;
+  const Plan base = c->plan;
+  if (family) *family = c->family[base.method][base.T];
+  if (nv == 0 && nf == 0) return MSH_OK;  // nothing to time
+  const size_t osz = base.Out == K_F32 ? 4 : 8;
+  const size_t vbytes = align256((size_t)nv * 3 * osz), fbytes = (size_t)nf * 3 * 8;
+  rc = grow(c, &c->ostage, &c->ostage_bytes, vbytes + fbytes + 256, "autotune output buffer");
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int best = -1;
+  float best_ms = 0.f;
+  for (int fam = 2; fam >= 0 && e == hipSuccess; --fam) {
+    Plan q = base;
+    q.family = fam;
+    VolArgs a = c->vol;
+    a.verts = c->ostage;
+    a.faces = (int64_t*)((char*)c->ostage + vbytes);
+    VolArgs probe = a;
+    if (variant_family(emit_variant(q, &probe)) != fam) continue;  // not for this volume
+    e = launch_emit(q, a, c->stream);  // warm-up
+    float ms = 0.f;
+    for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+      e = hipEventRecord(e0, c->stream);
+      if (e == hipSuccess) e = launch_emit(q, a, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e == hipSuccess && (rep == 0 || t < ms)) ms = t;
+    }
+    if (e == hipSuccess && (best < 0 || ms < best_ms)) {
+      best = fam;
+      best_ms = ms;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return hip_fail(c, e, "autotune");
+  c->family[base.method][base.T] = best;
+  c->plan.family = best;
+  if (family) *family = best;
   return MSH_OK;
 }
 

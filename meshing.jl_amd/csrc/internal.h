@@ -34,6 +34,10 @@ struct Plan {
   double epsA; // eps converted to A
   double omeA; // one(T) - eps, computed in promote(T, typeof(eps)), as A
   double a[3], b[3];  // axis endpoints in the axis element type
+  // emit kernel family: 2 lanes along x (k_emit_mc5 / k_emit_h5), 1 lanes
+  // along z (k_emit_mc4 / k_emit_halo), 0 the generic k_emit; -1 the
+  // default order (best first, falling back when a volume does not fit)
+  int family = -1;
 };
 
 struct Workspace {
@@ -87,6 +91,8 @@ VolArgs make_vol(const Plan& p, const Geom& g, const void* field, const Workspac
 // Emit kernel variant of a volume (volumes of one batched emit launch share
 // it) and its tiling; fills nti, ntj, ntiles.
 int emit_variant(const Plan& p, VolArgs* a);
+// the family (Plan::family) a variant belongs to
+int variant_family(int variant);
 void count_tiling(const Plan& p, VolArgs* a);
 
 hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream_t st);

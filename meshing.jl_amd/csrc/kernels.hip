@@ -813,31 +813,33 @@ static const int kShapeTX[4] = {31, 31, 15, 15}, kShapeTY[4] = {2, 1, 2, 1};
 // MSH_EMIT_V3=1 selects k_emit for every method, MSH_MC_KERNEL=4 k_emit_mc4
 // for cubes, MSH_HALO_KERNEL=4 k_emit_halo for tetrahedra and surface nets.
 int emit_variant(const Plan& p, VolArgs* a) {
+  int fam = p.family < 0 ? 2 : p.family;
+  // comparison runs: environment overrides of the family
   const char* legacy = getenv("MSH_EMIT_V3");
-  const bool lean = !(legacy && *legacy == '1');
+  if (legacy && *legacy == '1') fam = 0;
   const char* mck = getenv("MSH_MC_KERNEL");
-  if (lean && p.method == MC && !(mck && *mck == '4')) {
-    const int v = mc5_variant(p, a);
-    if (v >= 0) return 128 + v;
-  }
-  if (lean && p.method == MC) {
-    const int v = mc4_variant(p, a);
-    if (v >= 0) return v;
-  }
+  if (p.method == MC && mck && *mck == '4' && fam > 1) fam = 1;
   const char* hk = getenv("MSH_HALO_KERNEL");
-  if (lean && (p.method == MT || p.method == SN) && !(hk && *hk == '4')) {
-    const int v = h5_variant(p, a);
-    if (v >= 0) return 192 + v;
+  if (p.method != MC && hk && *hk == '4' && fam > 1) fam = 1;
+  if (fam >= 2) {
+    const int v = p.method == MC ? mc5_variant(p, a) : h5_variant(p, a);
+    if (v >= 0) return (p.method == MC ? 128 : 192) + v;
   }
-  if (lean && (p.method == MT || p.method == SN)) {
-    const int v = halo_variant(p, a);
-    if (v >= 0) return 64 + v;
+  if (fam >= 1) {
+    const int v = p.method == MC ? mc4_variant(p, a) : halo_variant(p, a);
+    if (v >= 0) return (p.method == MC ? 0 : 64) + v;
   }
   const int sh = generic_shape(p);
   a->nti = (a->g.nx - 1 + kShapeTX[sh] - 1) / kShapeTX[sh];
   a->ntj = (a->g.ny - 1 + kShapeTY[sh] - 1) / kShapeTY[sh];
   a->ntiles = (a->g.nx > 1 && a->g.ny > 1) ? (int64_t)a->nti * a->ntj * a->g.nzseg : 0;
   return 32 + sh;
+}
+
+int variant_family(int variant) {
+  if (variant >= 128) return 2;
+  if (variant >= 64 || variant < 32) return 1;
+  return 0;
 }
 
 void count_tiling(const Plan& p, VolArgs* a) { count4_tiling(p, a); }

@@ -1,0 +1,51 @@
+"""Every emit kernel family (lanes along x, lanes along z, generic) gives the
+oracle's mesh bit for bit, through the single and the batched entry points,
+and the autotuner picks one of them without changing results."""
+import numpy as np
+import pytest
+
+from helpers import assert_same_mesh, dev_field, noise, sphere_field
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("mc", np.float32), ("mc", np.float64), ("mt", np.float32), ("mt", np.float64),
+         ("sn", np.float32), ("sn", np.float64)]
+
+
+def method_of(pkg, name):
+    return {"mc": pkg.MarchingCubes(iso=0), "mt": pkg.MarchingTetrahedra(iso=0),
+            "sn": pkg.NaiveSurfaceNets(iso=0)}[name]
+
+
+@pytest.fixture
+def reset_families(pkg):
+    yield
+    for name, dt in CASES:
+        pkg.set_kernel_family(method_of(pkg, name), dt, -1)
+
+
+@pytest.mark.parametrize("family", [2, 1, 0])
+@pytest.mark.parametrize("name,dtype", CASES)
+def test_family_matches_oracle(pkg, oracle, reset_families, family, name, dtype):
+    m = method_of(pkg, name)
+    pkg.set_kernel_family(m, dtype, family)
+    fields = [noise((37, 29, 41), 3, dtype), sphere_field(30, 0.55, dtype)]
+    for f in fields:
+        V, F = pkg.isosurface(dev_field(f), m)
+        Vo, Fo = oracle.isosurface(f, name, iso=0)
+        assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
+    res = pkg.isosurface_batch([dev_field(f) for f in fields], m)
+    for f, (V, F) in zip(fields, res):
+        Vo, Fo = oracle.isosurface(f, name, iso=0)
+        assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
+
+
+@pytest.mark.parametrize("name,dtype", CASES)
+def test_autotune_keeps_results(pkg, oracle, reset_families, name, dtype):
+    m = method_of(pkg, name)
+    f = noise((70, 66, 60), 5, dtype)
+    fam = pkg.autotune(dev_field(f), m)
+    assert fam in (0, 1, 2)
+    V, F = pkg.isosurface(dev_field(f), m)
+    Vo, Fo = oracle.isosurface(f, name, iso=0)
+    assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
