@@ -17,7 +17,8 @@ Printed: one JSON line.  `value` is Gvoxels/s (samples per second / 1e9).
 k_emit_mc5), from HIP events on the stream the library launches on; its
 `traffic` is the HBM bytes per launch from the rocprofv3 FETCH_SIZE and
 WRITE_SIZE summaries kept in profiles/ (null when absent).  `cpu_baseline`
-times the CPU oracle (single thread) on an x-slab of the same field.  A
+times the CPU oracle (single thread) on an x-slab of the same field, and
+`cpu_baseline_all_cores` the same sample on up to 16 threads.  A
 1 GiB device-to-device copy is timed at the end as the practical HBM rate
 of the same box (and is the known-size kernel the counter scale factor is
 read from).
@@ -136,7 +137,7 @@ def measure(m, dtype, n, method_name, steps, warmup, kind="value_noise"):
     field_bytes = N * esz
     out_bytes = r.nv * 3 * osz + r.nf * 3 * 8
     res = dict(dtype=dtype, samples=N, nverts=r.nv, nfaces=r.nf, step_s=dt, count_s=t_count,
-               family=m.KERNEL_FAMILIES[family],
+               family=m.KERNEL_FAMILIES[family], family_id=family,
                emit_s=t_emit, field_bytes=field_bytes, out_bytes=out_bytes,
                gvox_s=N / dt / 1e9, tris_s=r.nf / dt)
     del r, field
@@ -388,10 +389,23 @@ def cpu_baseline(m, n, planes):
     t0 = time.perf_counter()
     V, F = O.isosurface(slab, "mc", iso=0)
     dt = time.perf_counter() - t0
-    return dict(value=slab.size / dt / 1e9, unit="Gvoxels/s", cores=1, kind="port",
-                sample=f"MarchingCubes(iso=0) on x-planes 1..{planes} of the Float64 {n}^3 "
-                       f"value-noise field ({slab.size} samples, {len(F)} faces), CPU oracle "
-                       f"(single-threaded restatement), {dt:.2f} s")
+    nf = len(F)
+    del V, F
+    one = dict(value=slab.size / dt / 1e9, unit="Gvoxels/s", cores=1, kind="port",
+               sample=f"MarchingCubes(iso=0) on x-planes 1..{planes} of the Float64 {n}^3 "
+                      f"value-noise field ({slab.size} samples, {nf} faces), CPU oracle "
+                      f"(single-threaded restatement), {dt:.2f} s")
+    # the same sample over all host cores this process may use (x-slabs on
+    # threads; at most 16, the CPU share of one GPU)
+    nt = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    V, F = O.isosurface(slab, "mc", iso=0, threads=nt)
+    dtt = time.perf_counter() - t0
+    assert len(F) == nf
+    del V, F
+    allc = dict(value=slab.size / dtt / 1e9, unit="Gvoxels/s", cores=nt, kind="port",
+                sample=f"as cpu_baseline, x-slabs of the CPU oracle on {nt} threads, {dtt:.2f} s")
+    return one, allc
 
 
 def main():
@@ -420,8 +434,11 @@ def main():
     serve = (measure_serve(m, a.serve_requests, [float(q) for q in a.serve_qps.split(",")])
              if a.serve_requests else None)
     copy = copy_probe()
-    cpu = None if a.no_cpu else cpu_baseline(m, a.n, a.cpu_planes)
-    kname = {"mc": "k_emit_mc5", "mt": "k_emit_halo (tetrahedra)", "sn": "k_emit_halo (surface nets)"}[a.method]
+    cpu, cpu_all = (None, None) if a.no_cpu else cpu_baseline(m, a.n, a.cpu_planes)
+    # the emit kernel each autotuned family launches
+    knames = {"mc": {2: "k_emit_mc5", 1: "k_emit_mc4", 0: "k_emit"},
+              "mt": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"},
+              "sn": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"}}[a.method]
 
     def roofline(r):
         b = r["field_bytes"] + r["out_bytes"]
@@ -429,7 +446,7 @@ def main():
         return {"bound": "hbm", "achieved": round(ach / 1e9, 2), "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4),
                 "traffic": traffic_of(a.method, a.n, r["dtype"]),
-                "kernel": kname, "algorithmic_bytes_per_launch": b,
+                "kernel": knames.get(r["family_id"], "k_emit"), "algorithmic_bytes_per_launch": b,
                 "avg_launch_ms": round(r["emit_s"] * 1e3, 4)}
 
     def summary(r):
@@ -461,6 +478,7 @@ def main():
                    "shape": [a.n] * 3, "seed": 1},
         "roofline": roofline(main_r),
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "hbm_copy_probe": copy,
         "gyroid_256_f32": gyroid,
         "batched": batch,

@@ -34,6 +34,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -163,11 +164,16 @@ struct Result {
 
 struct Pt { double c[3]; };
 
+// Cells with xi in [xlo, xhi] (default: all).  Face indices count from the
+// first vertex this call emits, so a slab's result is the serial result's
+// slice with its faces shifted by the vertices of the slabs before it.
 template <class T, class I, class Out>
-void mc_isosurface(const Grid<T>& sdf, I iso, const Axis (&ax)[3], Result<Out>& r) {
+void mc_isosurface(const Grid<T>& sdf, I iso, const Axis (&ax)[3], Result<Out>& r,
+                   int64_t xlo = 1, int64_t xhi = -1) {
   const int64_t nx = sdf.nx, ny = sdf.ny, nz = sdf.nz;
   const Axis &xp = ax[0], &yp = ax[1], &zp = ax[2];
-  for (int64_t xi = 1; xi <= nx - 1; ++xi)
+  if (xhi < 0) xhi = nx - 1;
+  for (int64_t xi = xlo; xi <= xhi; ++xi)
     for (int64_t yi = 1; yi <= ny - 1; ++yi)
       for (int64_t zi = 1; zi <= nz - 1; ++zi) {
         const T iso_vals[8] = {sdf(xi, yi, zi),         sdf(xi + 1, yi, zi),
@@ -433,9 +439,42 @@ int out_kind_of(int dtype, const int* kinds, const Scalar& iso, const Scalar* ep
   return f64 ? K_F64 : K_F32;
 }
 
+int g_threads = 1;  // > 1: Marching Cubes over x-slabs on that many threads
+
+// The multi-threaded form of mc_isosurface (a CPU baseline over all cores,
+// not a restatement): x-slabs on separate threads, concatenated in x order
+// with each slab's faces shifted by the vertices before it, which is the
+// serial result exactly because a cube's faces name only its own vertices.
+template <class T, class I, class Out>
+void mc_isosurface_threads(const Grid<T>& g, I iso, const Axis (&ax)[3], Result<Out>& r,
+                           int nt) {
+  const int64_t ncx = g.nx - 1;
+  if (ncx <= 0) return;
+  if (nt > ncx) nt = (int)ncx;
+  std::vector<Result<Out>> part(nt);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      const int64_t lo = 1 + ncx * t / nt, hi = ncx * (t + 1) / nt;
+      mc_isosurface<T, I, Out>(g, iso, ax, part[t], lo, hi);
+    });
+  for (auto& x : th) x.join();
+  size_t nv = 0, nf = 0;
+  for (auto& p : part) nv += p.v.size(), nf += p.f.size();
+  r.v.reserve(nv);
+  r.f.reserve(nf);
+  for (auto& p : part) {
+    const int64_t shift = (int64_t)(r.v.size() / 3);
+    r.v.insert(r.v.end(), p.v.begin(), p.v.end());
+    for (int64_t i : p.f) r.f.push_back(i + shift);
+    Result<Out>().v.swap(p.v);
+  }
+}
+
 template <class T, class I, class E, class Out>
 void run_method(int method, const Grid<T>& g, I iso, E eps, const Axis (&ax)[3], Result<Out>& r) {
-  if (method == 0) mc_isosurface<T, I, Out>(g, iso, ax, r);
+  if (method == 0 && g_threads > 1) mc_isosurface_threads<T, I, Out>(g, iso, ax, r, g_threads);
+  else if (method == 0) mc_isosurface<T, I, Out>(g, iso, ax, r);
   else if (method == 1) MT<T, I, E, Out>(g, iso, eps, ax, r).run();
   else sn_isosurface<T, I, Out>(g, iso, ax, r);
 }
@@ -515,6 +554,11 @@ int oracle_isosurface(int method, int dtype, const void* sdf, int64_t nx, int64_
     return -1;
   }
 }
+
+// Threads used by later oracle_isosurface calls for Marching Cubes (1:
+// the single-threaded restatement; MT and SN always run on one thread).
+// A process-wide setting of this test library, not for concurrent use.
+void oracle_set_threads(int n) { g_threads = n > 1 ? n : 1; }
 
 void oracle_free(void* p) { std::free(p); }
 const char* oracle_last_error(void) { return g_err.c_str(); }

@@ -36,6 +36,7 @@ def lib():
                                         ctypes.POINTER(I)]
         L.oracle_isosurface.restype = I
         L.oracle_free.argtypes = [P]
+        L.oracle_set_threads.argtypes = [I]
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_linrange.argtypes = [I, D, D, I64, P]
         L.oracle_linrange.restype = I
@@ -70,7 +71,10 @@ def _endpoints(R):
     return a, b
 
 
-def isosurface(sdf, method="mc", iso=0.0, eps=1e-3, X=(-1, 1), Y=(-1, 1), Z=(-1, 1)):
+def isosurface(sdf, method="mc", iso=0.0, eps=1e-3, X=(-1, 1), Y=(-1, 1), Z=(-1, 1),
+               threads=1):
+    """threads > 1 runs Marching Cubes over x-slabs on that many threads (a
+    CPU baseline over all cores; same result as the single-threaded form)."""
     sdf = np.asarray(sdf)
     if sdf.ndim != 3:
         raise ValueError("sdf must be 3-D")
@@ -98,6 +102,7 @@ def isosurface(sdf, method="mc", iso=0.0, eps=1e-3, X=(-1, 1), Y=(-1, 1), Z=(-1,
     vp, fp = ctypes.c_void_p(), ctypes.c_void_p()
     nv, nf, ok = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
     L = lib()
+    L.oracle_set_threads(int(threads))
     rc = L.oracle_isosurface(METHODS[method] if isinstance(method, str) else int(method), dtype,
                              a.ctypes.data, nx, ny, nz, ik, float(iso), iso_i, ek, float(eps),
                              eps_i, kinds, lo, hi, ctypes.byref(vp), ctypes.byref(nv),

@@ -272,3 +272,17 @@ def test_empty_and_degenerate(oracle):
         assert len(V) == 0 and len(F) == 0
     V, F = oracle.isosurface(np.ones((5, 5, 5)), "mt", iso=0)
     assert len(V) == 0
+
+
+@pytest.mark.parametrize("dtype,shape,threads", [(np.float64, (23, 9, 11), 4),
+                                                  (np.float32, (7, 12, 10), 16),
+                                                  (np.float64, (2, 5, 5), 3)])
+def test_mc_threaded_equals_serial(oracle, dtype, shape, threads):
+    # the multi-threaded CPU baseline (x-slabs on threads) returns the
+    # single-threaded result exactly, including more threads than x-cells
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(shape).astype(dtype)
+    V1, F1 = oracle.isosurface(f, "mc", iso=0.05)
+    Vt, Ft = oracle.isosurface(f, "mc", iso=0.05, threads=threads)
+    assert len(F1) > 0
+    assert np.array_equal(V1, Vt) and np.array_equal(F1, Ft)
