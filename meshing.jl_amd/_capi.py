@@ -67,6 +67,13 @@ def lib():
                 raise RuntimeError(
                     f"{LIB_PATH} is not built; run `make -C {LIB_PATH.parent}` "
                     "(hipcc, gfx950).  There is no CPU fallback.")
+            # torch ships its own HIP runtime under the same soname; load it
+            # first so that the library binds to it and the process has one
+            # HIP runtime whichever side touches the GPU first
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             L = ctypes.CDLL(str(LIB_PATH))
             vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
             pf, pp = ctypes.POINTER(MshField), ctypes.POINTER(MshParams)

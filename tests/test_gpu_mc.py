@@ -103,3 +103,18 @@ def test_repeated_calls_grow_and_shrink(pkg, oracle):
 def test_value_noise_256(pkg, oracle):
     f = pkg.generate_field("value_noise", (256, 256, 256), "float32", seed=3)
     run_both(pkg, oracle, f.cpu().numpy(), 0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_dense_crossings(pkg, oracle, dtype):
+    # white noise: nearly every cell crosses, so chunks overflow the emit
+    # queue and are taken one z step at a time
+    f = np.random.default_rng(11).standard_normal((70, 40, 90)).astype(dtype)
+    run_both(pkg, oracle, f, 0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_long_z_rows(pkg, oracle, dtype):
+    # long z rows, split into z segments (units) by the host
+    f = noise((9, 7, 1500), 4, dtype)
+    run_both(pkg, oracle, f, 0.02)
