@@ -49,6 +49,7 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
     uint8_t cc[kWaves][K ? ZC : 1][64];  // case of each cell of the chunk
     uint8_t pc[kWaves][K ? 64 : 1];      // case of the cell just below the chunk
     uint32_t edge[19];                // per edge: corner offsets and axis bits
+    uint16_t xmask[K ? 1 : 256];      // surface nets: crossing cube edges, by natural case
     uint16_t cp[kWaves][ZC][64];      // in-row vertex prefix of each cell of the chunk
     int64_t rb[kWaves][64];           // row vertex offset at the chunk start
     int64_t dv[kWaves][64], df[kWaves][64];  // per row: output index minus queue slot
@@ -113,6 +114,17 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
     sm.edge[threadIdx.x] = oa | (ob << 10) | ((uint32_t)(xa | (ya << 1) | (za << 2)) << 20) |
                            ((uint32_t)(xb | (yb << 1) | (zb << 2)) << 23) |
                            (METHOD == SN ? ((uint32_t)ca << 26) | ((uint32_t)cb << 29) : 0u);
+  }
+  if constexpr (METHOD == SN) {
+    for (int c = threadIdx.x; c < 256; c += kBlock) {
+      const uint32_t csm = nat_to_mc((uint32_t)c);
+      uint32_t m = 0;
+      for (int ed = 0; ed < 12; ++ed) {
+        const uint32_t ew = MC_EDGE[ed];
+        m |= (((csm >> (ew & 0xF)) ^ (csm >> (ew >> 4))) & 1u) << ed;
+      }
+      sm.xmask[c] = (uint16_t)m;
+    }
   }
   for (int q = threadIdx.x; q < SX; q += kBlock) sm.cx[q] = a.coords[min(max(x0 + q, 0), g.nx - 1)];
   if (threadIdx.x < SY)
@@ -274,16 +286,18 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
             // the mean of the crossing points of the cell's 12 edges, each
             // interpolated as vertex_interp does (src/marching_cubes.jl:100-104),
             // summed per axis in promote(M, axis type)
-            const uint32_t csm = nat_to_mc(ecs);
             float fx = 0.f, fy = 0.f, fz = 0.f;
             double dx = 0.0, dy = 0.0, dz = 0.0;
             const bool sx = sizeof(M) == 4 && cx32, sy = sizeof(M) == 4 && cy32,
                        sz = sizeof(M) == 4 && cz32;
-            int ncross = 0;
-            for (int ed = 0; ed < 12; ++ed) {
+            // the crossing edges in edge order (the order of the sum), one
+            // iteration each rather than a pass over all 12
+            uint32_t xm = sm.xmask[ecs];
+            const int ncross = __popc(xm);
+            while (xm != 0) {
+              const int ed = __builtin_ctz(xm);
+              xm &= xm - 1;
               const uint32_t ew = sm.edge[ed];
-              if (((csm >> ((ew >> 26) & 7)) & 1) == ((csm >> (ew >> 29)) & 1)) continue;
-              ++ncross;
               const T va = *(const T*)(cell + (ew & 0x3FF)), vb = *(const T*)(cell + ((ew >> 10) & 0x3FF));
               const M mu = (iso - (M)va) / (M)(vb - va);
               const double qx = lerp_axis<M>(cx32, mu, sm.cx[lc + ((ew >> 20) & 1)], sm.cx[lc + ((ew >> 23) & 1)]);
@@ -490,11 +504,15 @@ __global__ __launch_bounds__(kBlock) void k_emit_h5_b(const VolArgs* __restrict_
 // ---------------------------------------------------------------------------
 // Host side.
 
-template <typename T>
-static constexpr int h5_zc() { return sizeof(T) == 4 ? 8 : 4; }
+// z chunk: 8 cells, and 4 for tetrahedra over Float64 (whose LDS tables and
+// brick would not fit twice per CU at 8); 16 measured slower for Float32
+// surface nets on dense fields (more chunks overflow the queue)
+template <int METHOD, typename T>
+static constexpr int h5_zc() { return (sizeof(T) == 8 && METHOD == MT) ? 4 : 8; }
 
 int h5_variant(const Plan& p, VolArgs* a) {
-  const int zc = p.T == K_F32 ? h5_zc<float>() : h5_zc<double>();
+  const int zc = p.method == SN ? (p.T == K_F32 ? h5_zc<SN, float>() : h5_zc<SN, double>())
+                                : (p.T == K_F32 ? h5_zc<MT, float>() : h5_zc<MT, double>());
   const int64_t esz = p.T == K_F32 ? 4 : 8;
   // the brick's 32-bit DMA offsets reach plane zc+1, row 4, sample 63
   if (((int64_t)(zc + 1) * a->g.nx * a->g.ny + 4ll * a->g.nx + 64) * esz >= (1ll << 31)) return -1;
@@ -511,7 +529,7 @@ int h5_variant(const Plan& p, VolArgs* a) {
 template <int METHOD, typename T, typename M, typename A, typename Out, int CXM>
 static hipError_t h5_z(bool vec, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
                        int64_t gbase, int64_t nblocks, hipStream_t st) {
-  constexpr int ZC = h5_zc<T>();
+  constexpr int ZC = h5_zc<METHOD, T>();
   if (vols) {
     if (vec)
       hipLaunchKernelGGL((k_emit_h5_b<METHOD, T, M, A, Out, ZC, true, CXM>), dim3((unsigned)nblocks),
