@@ -3,14 +3,14 @@
 // sn_isosurface) emit kernels with lanes along x.
 //
 // The layout of the Marching Cubes kernel with lanes along x (mc5.hip): lane
-// i of wave w holds the cell row (x0 + i, jb - 1 + w) and marches it through
-// the z chunk with running in-row prefixes.  A face of these methods names
+// i of wave w holds the cell row (x0 + i, jb + w) and marches it through the
+// z chunk with running in-row prefixes.  A face of these methods names
 // vertices owned by the cells at -x, -y and -z, so the block classifies one
-// more row and column than it emits: wave 0 holds the row at -y (it only
-// classifies), lane 3 the column at -x.  Every cell's case and in-row vertex
-// prefix go to LDS, a barrier makes them visible, and the emitting waves
-// (1..3) look the owners up there; the cell just below the chunk comes from
-// one extra plane of the brick.
+// more row and column than it emits: wave 0 also classifies the row at -y,
+// lane 3 holds the column at -x.  Every cell's case and in-row vertex prefix
+// go to LDS (row slot 0 is the row at -y, slot w + 1 wave w's row), a
+// barrier makes them visible, and the waves look the owners up there; the
+// cell just below the chunk comes from one extra plane of the brick.
 #include <stdlib.h>
 
 #include <type_traits>
@@ -24,13 +24,14 @@ namespace msh {
 namespace {
 
 constexpr int kH5TX = 56;  // cells a block emits along x: lanes 4..59
-constexpr int kH5CY = 3;   // cell rows a block emits along y: waves 1..3
+constexpr int kH5CY = 4;   // cell rows a block emits along y: one per wave
 
 template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM>
 __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
   static_assert(METHOD == MT || METHOD == SN, "tetrahedra or surface nets");
   static_assert(ZC <= 16, "z fits 4 bits of a queue entry");
-  constexpr int SX = 64, SY = 5, NP = ZC + 2;  // planes zc-1 .. zc+ZC, rows jb-1 .. jb+3
+  constexpr int SX = 64, SY = kH5CY + 2, NP = ZC + 2;  // planes zc-1 .. zc+ZC, rows jb-1 .. jb+4
+  constexpr int NS = kH5CY + 1;  // row slots: the row at -y, then one per wave
   constexpr int DW = (int)(sizeof(T) / 4), RD = SX * DW;
   constexpr int CAPE = 128, CAPV = 1024, CAPF = 1024;  // per wave
   static_assert(kH5TX * 13 <= CAPV && kH5TX * 12 <= CAPF && kH5TX <= CAPE, "one z step must fit");
@@ -46,12 +47,12 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
     uint8_t rank0[K ? 256 * 19 : 1];  // rank of each edge among a class-0 voxel's vertices
     uint8_t own0[K ? 256 * 8 : 1];    // edges of a class-0 voxel's vertices, in order
     uint8_t shift[K ? 152 : 1];       // MT_SHIFT
-    uint8_t cc[kWaves][K ? ZC : 1][64];  // case of each cell of the chunk
-    uint8_t pc[kWaves][K ? 64 : 1];      // case of the cell just below the chunk
+    uint8_t cc[NS][K ? ZC : 1][64];   // case of each cell of the chunk
+    uint8_t pc[NS][K ? 64 : 1];       // case of the cell just below the chunk
     uint32_t edge[19];                // per edge: corner offsets and axis bits
     uint16_t xmask[K ? 1 : 256];      // surface nets: crossing cube edges, by natural case
-    uint16_t cp[kWaves][ZC][64];      // in-row vertex prefix of each cell of the chunk
-    int64_t rb[kWaves][64];           // row vertex offset at the chunk start
+    uint16_t cp[NS][ZC][64];          // in-row vertex prefix of each cell of the chunk
+    int64_t rb[NS][64];               // row vertex offset at the chunk start
     int64_t dv[kWaves][64], df[kWaves][64];  // per row: output index minus queue slot
     // queue entries: x | z << 6 | case << 10 | class << 18 | first slot << 21
     uint32_t vlist[kWaves][CAPE], flist[kWaves][CAPE];
@@ -130,16 +131,20 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
   if (threadIdx.x < SY)
     sm.cy[threadIdx.x] = a.coords[g.nx + min(max(y0 + (int)threadIdx.x, 0), g.ny - 1)];
 
-  // this lane's cell row
-  const int gi = x0 + lane, gj = y0 + wv;
-  const bool exists = lane < SX - 1 && gi >= 0 && gi < g.nx - 1 && gj >= 0 && gj < g.ny - 1;
-  const bool emits = exists && wv >= 1 && lane >= 4 && lane < 4 + kH5TX;
-  int64_t rbv = 0, rbf = 0;
+  // this lane's cell row (row slot sl), and for wave 0 the row at -y
+  const int sl = wv + 1;
+  const int gi = x0 + lane, gj = y0 + sl;
+  const bool colok = lane < SX - 1 && gi >= 0 && gi < g.nx - 1;
+  const bool exists = colok && gj < g.ny - 1;
+  const bool emits = exists && lane >= 4 && lane < 4 + kH5TX;
+  const bool hexists = wv == 0 && colok && y0 >= 0;
+  int64_t rbv = 0, rbf = 0, hrbv = 0;
   if (exists) {
     const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + gj) * g.nzseg + s;
     rbv = a.voff[u] - vbase;
     rbf = a.foff[u] - fbase;
   }
+  if (hexists) hrbv = a.voff[a.ubase + ((int64_t)gi * (g.ny - 1) + y0) * g.nzseg + s] - vbase;
 
   // DMA: planes zc-1 .. zc+ZC, rows y0 .. y0+4, 64 samples from x0.  Near
   // the volume's low faces the origin is outside the field: the buffer starts
@@ -212,8 +217,8 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
     uint32_t nib[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-      const T v0 = *(const T*)(smp + (p * SY + wv) * RD + lane * DW);
-      const T v1 = *(const T*)(smp + (p * SY + wv + 1) * RD + lane * DW);
+      const T v0 = *(const T*)(smp + (p * SY + sl) * RD + lane * DW);
+      const T v1 = *(const T*)(smp + (p * SY + sl + 1) * RD + lane * DW);
       const uint32_t w = (uint32_t)(v0 < thr) | ((uint32_t)(v1 < thr) << 2);
       nib[p] = w | (next_lane(w, 0u) << 1);
     }
@@ -228,7 +233,7 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
       if constexpr (METHOD == MT) {
         nvz[z] = sm.nown[cls * 256 + cs[z]];
         nfz[z] = emits ? (uint32_t)sm.nf[cs[z]] : 0u;
-        sm.cc[wv][z][lane] = (uint8_t)cs[z];
+        sm.cc[sl][z][lane] = (uint8_t)cs[z];
       } else {
         const uint32_t csm = nat_to_mc(cs[z]), c0 = csm & 1;
         nvz[z] = (cs[z] != 0 && cs[z] != 255) ? 1u : 0u;
@@ -237,18 +242,48 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
                             (uint32_t)((c0 != ((csm >> 4) & 1)) && gi >= 1 && gj >= 1);
         nfz[z] = (emits && nvz[z]) ? 2u * nq : 0u;
       }
-      sm.cp[wv][z][lane] = (uint16_t)runv;
+      sm.cp[sl][z][lane] = (uint16_t)runv;
       runv += nvz[z];
       if (emits) tot += nvz[z] | (nfz[z] << 16);
     }
-    if constexpr (METHOD == MT) sm.pc[wv][lane] = (uint8_t)((exists && zc > 0) ? (nib[0] | (nib[1] << 4)) : 0u);
-    sm.rb[wv][lane] = rbv;
+    if constexpr (METHOD == MT) sm.pc[sl][lane] = (uint8_t)((exists && zc > 0) ? (nib[0] | (nib[1] << 4)) : 0u);
+    sm.rb[sl][lane] = rbv;
+    if (wv == 0) {
+      // the row at -y (slot 0): cases and vertex prefixes for the owner
+      // lookups of row jb's faces; it emits nothing here
+      uint32_t hn[NP];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const T v0 = *(const T*)(smp + (p * SY) * RD + lane * DW);
+        const T v1 = *(const T*)(smp + (p * SY + 1) * RD + lane * DW);
+        const uint32_t w = (uint32_t)(v0 < thr) | ((uint32_t)(v1 < thr) << 2);
+        hn[p] = w | (next_lane(w, 0u) << 1);
+      }
+      uint32_t hrun = 0;
+#pragma unroll
+      for (int z = 0; z < ZC; ++z) {
+        const uint32_t c = (hexists && z < zn) ? (hn[z + 1] | (hn[z + 2] << 4)) : 0u;
+        uint32_t nv;
+        if constexpr (METHOD == MT) {
+          const uint32_t cls = (uint32_t)((gi == 0) | ((y0 == 0) << 1) | ((zc + z == 0) << 2));
+          nv = sm.nown[cls * 256 + c];
+          sm.cc[0][z][lane] = (uint8_t)c;
+        } else {
+          nv = (c != 0 && c != 255) ? 1u : 0u;
+        }
+        sm.cp[0][z][lane] = (uint16_t)hrun;
+        hrun += nv;
+      }
+      if constexpr (METHOD == MT) sm.pc[0][lane] = (uint8_t)((hexists && zc > 0) ? (hn[0] | (hn[1] << 4)) : 0u);
+      sm.rb[0][lane] = hrbv;
+      hrbv += hrun;
+    }
     // this wave's share of the next brick must land before the barrier that
     // ends the chunk; wait now, before the queue's stores are issued
     __builtin_amdgcn_s_waitcnt(0);
     lds_barrier();  // every row's cases and prefixes are in LDS
 
-    if (wv >= 1) {
+    {
       int nvl = 0, nfl = 0;
       uint32_t V = 0, F = 0;
       auto flush = [&]() {
@@ -264,7 +299,7 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
           const int lc = (int)(e & 63), c = (int)((e >> 6) & 15);
           const uint32_t ecs = (e >> 10) & 0xFF, cls = (e >> 18) & 7, sv = e >> 21;
           const uint32_t q = o - sv;
-          const uint32_t* cell = smp + ((c + 1) * SY + wv) * RD + lc * DW;
+          const uint32_t* cell = smp + ((c + 1) * SY + sl) * RD + lc * DW;
           double px, py, pz;
           if constexpr (METHOD == MT) {
             // class-0 voxels (all but the first layers) read the LDS copy
@@ -280,7 +315,7 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
             const A wy = (A)((ew >> 21) & 1) * bw + (A)((ew >> 24) & 1) * aw;
             const A wz = (A)((ew >> 22) & 1) * bw + (A)((ew >> 25) & 1) * aw;
             px = mt_axis<A>(cx32, wx, sm.cx[lc], sm.cx[lc + 1]);
-            py = mt_axis<A>(cy32, wy, sm.cy[wv], sm.cy[wv + 1]);
+            py = mt_axis<A>(cy32, wy, sm.cy[sl], sm.cy[sl + 1]);
             pz = mt_axis<A>(cz32, wz, czs[c + 1], czs[c + 2]);
           } else {
             // the mean of the crossing points of the cell's 12 edges, each
@@ -301,7 +336,7 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
               const T va = *(const T*)(cell + (ew & 0x3FF)), vb = *(const T*)(cell + ((ew >> 10) & 0x3FF));
               const M mu = (iso - (M)va) / (M)(vb - va);
               const double qx = lerp_axis<M>(cx32, mu, sm.cx[lc + ((ew >> 20) & 1)], sm.cx[lc + ((ew >> 23) & 1)]);
-              const double qy = lerp_axis<M>(cy32, mu, sm.cy[wv + ((ew >> 21) & 1)], sm.cy[wv + ((ew >> 24) & 1)]);
+              const double qy = lerp_axis<M>(cy32, mu, sm.cy[sl + ((ew >> 21) & 1)], sm.cy[sl + ((ew >> 24) & 1)]);
               const double qz = lerp_axis<M>(cz32, mu, czs[c + 1 + ((ew >> 22) & 1)], czs[c + 1 + ((ew >> 25) & 1)]);
               if (sx) fx = fx + (float)qx; else dx = dx + qx;
               if (sy) fy = fy + (float)qy; else dy = dy + qy;
@@ -345,7 +380,7 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
               const int dy = (((ew >> 24) & 1) == 0) & (cj > 0);
               const int dz = (((ew >> 25) & 1) == 0) & (ck > 0);
               const uint32_t e2 = sm.shift[ed * 8 + (dx | (dy << 1) | (dz << 2))];
-              const int l2 = lc - dx, w2 = wv - dy, z2 = c - dz;
+              const int l2 = lc - dx, w2 = sl - dy, z2 = c - dz;
               const uint32_t cls2 =
                   (uint32_t)((ci - dx == 0) | ((cj - dy == 0) << 1) | ((ck - dz == 0) << 2));
               uint32_t oc;
@@ -378,11 +413,11 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
               const int d0 = (u == 0 ? du : 0) + (w == 0 ? dw : 0);
               const int d1 = (u == 1 ? du : 0) + (w == 1 ? dw : 0);
               const int d2 = (u == 2 ? du : 0) + (w == 2 ? dw : 0);
-              const int l2 = lc - d0, w2 = wv - d1, z2 = c - d2;
+              const int l2 = lc - d0, w2 = sl - d1, z2 = c - d2;
               if (z2 >= 0) return sm.rb[w2][l2] + sm.cp[w2][z2][l2] + 1;
               return sm.rb[w2][l2];  // the cell just below the chunk, active: the row's last vertex
             };
-            const int64_t me = sm.rb[wv][lc] + sm.cp[wv][c][lc] + 1;
+            const int64_t me = sm.rb[sl][lc] + sm.cp[sl][c][lc] + 1;
             const int64_t q1 = cell_id(1, 0), q2 = cell_id(1, 1), q3 = cell_id(0, 1);
             const int64_t a1 = c0 ? q1 : q3, a3 = c0 ? q3 : q1;
             id[0] = me;
@@ -514,8 +549,10 @@ int h5_variant(const Plan& p, VolArgs* a) {
   const int zc = p.method == SN ? (p.T == K_F32 ? h5_zc<SN, float>() : h5_zc<SN, double>())
                                 : (p.T == K_F32 ? h5_zc<MT, float>() : h5_zc<MT, double>());
   const int64_t esz = p.T == K_F32 ? 4 : 8;
-  // the brick's 32-bit DMA offsets reach plane zc+1, row 4, sample 63
-  if (((int64_t)(zc + 1) * a->g.nx * a->g.ny + 4ll * a->g.nx + 64) * esz >= (1ll << 31)) return -1;
+  // the brick's 32-bit DMA offsets reach plane zc+1, row kH5CY+1, sample 63
+  if (((int64_t)(zc + 1) * a->g.nx * a->g.ny + (int64_t)(kH5CY + 1) * a->g.nx + 64) * esz >=
+      (1ll << 31))
+    return -1;
   a->nti = (a->g.nx - 1 + kH5TX - 1) / kH5TX;
   a->ntj = (a->g.ny - 1 + kH5CY - 1) / kH5CY;
   a->ntiles = (a->g.nx > 1 && a->g.ny > 1) ? (int64_t)a->nti * a->ntj * a->g.nzseg : 0;
