@@ -14,7 +14,8 @@ for any tensor shared across streams.
 The scheduler thread forms a batch from every queued request that fits the
 HBM budget (fields + workspace + an output estimate, corrected to the exact
 output size once the batch is counted), runs ``msh_batch_count`` and
-``msh_batch_extract`` on one of two HIP streams, and moves on: the next
+``msh_batch_extract`` (``msh_count`` and ``msh_extract`` for a batch of one)
+on one of two HIP streams, and moves on: the next
 batch's classify/count runs on the other stream while this batch's emit is
 still on the GPU.  A completion thread waits for each batch's event,
 resolves its futures and returns its memory to the budget, which lets the
@@ -210,7 +211,13 @@ class Server:
                     nv = (ctypes.c_int64 * n)()
                     nf = (ctypes.c_int64 * n)()
                     t0 = time.perf_counter()
-                    ctx.check(self.L.msh_batch_count(ctx.handle, n, F, P, nv, nf))
+                    # a lone request takes the single-volume entry points
+                    # (same result, fewer launches and uploads)
+                    if n == 1:
+                        ctx.check(self.L.msh_count(ctx.handle, ctypes.byref(F[0]),
+                                                   ctypes.byref(P[0]), nv, nf))
+                    else:
+                        ctx.check(self.L.msh_batch_count(ctx.handle, n, F, P, nv, nf))
                     t1 = time.perf_counter()
                     vp = (ctypes.c_void_p * n)()
                     fp = (ctypes.c_void_p * n)()
@@ -232,7 +239,12 @@ class Server:
                     with self.lock:
                         self.in_use += extra
                     t2 = time.perf_counter()
-                    ctx.check(self.L.msh_batch_extract(ctx.handle, n, F, P, vp, fp, MSH_DEVICE))
+                    if n == 1:
+                        ctx.check(self.L.msh_extract(ctx.handle, ctypes.byref(F[0]),
+                                                     ctypes.byref(P[0]), vp[0], fp[0], MSH_DEVICE))
+                    else:
+                        ctx.check(self.L.msh_batch_extract(ctx.handle, n, F, P, vp, fp,
+                                                           MSH_DEVICE))
                     ev = torch.cuda.Event()
                     ev.record(stream)
                     t3 = time.perf_counter()
