@@ -436,7 +436,7 @@ def main():
     copy = copy_probe()
     cpu, cpu_all = (None, None) if a.no_cpu else cpu_baseline(m, a.n, a.cpu_planes)
     # the emit kernel each autotuned family launches
-    knames = {"mc": {2: "k_emit_mc5", 1: "k_emit_mc4", 0: "k_emit"},
+    knames = {"mc": {3: "k_emit_mc5", 2: "k_emit_mc5", 1: "k_emit_mc4", 0: "k_emit"},
               "mt": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"},
               "sn": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"}}[a.method]
 

@@ -110,9 +110,12 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
 int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
                       void* const* verts, int64_t* const* faces, int32_t out_memory);
 
-/* Kernel choice.  The emit pass has three kernel families per method: 2
- * lanes along x (the default), 1 lanes along z, 0 a generic kernel; each
- * family falls back to the next one down for volumes it cannot take.
+/* Kernel choice.  The emit pass has kernel families per method: 2 lanes
+ * along x (the default), 1 lanes along z, 0 a generic kernel, and 3 the
+ * lanes-along-x kernel with half-depth bricks (a tile shape; Marching Cubes
+ * with one element type throughout and Float64 axes, otherwise the same as
+ * 2); each family falls back to the next one down for volumes it cannot
+ * take.
  * msh_autotune counts the given field, times the emit of every family that
  * takes it (three launches each on the context's stream, into a scratch
  * buffer) and records the fastest for the field's (method, element type) in

@@ -37,7 +37,8 @@ __all__ = ["isosurface", "isosurface_batch", "AbstractMeshingAlgorithm", "Marchi
            "autotune", "set_kernel_family", "KERNEL_FAMILIES", "Server", "FieldSpec"]
 
 # emit kernel families, best first (see msh_autotune in include/meshing_hip.h)
-KERNEL_FAMILIES = {2: "lanes along x", 1: "lanes along z", 0: "generic", -1: "default"}
+KERNEL_FAMILIES = {3: "lanes along x, half-depth bricks", 2: "lanes along x",
+                   1: "lanes along z", 0: "generic", -1: "default"}
 
 
 class AbstractMeshingAlgorithm:
@@ -339,8 +340,8 @@ def autotune(sdf, method=None, *args, X=(-1, 1), Y=(-1, 1), Z=(-1, 1), device=0)
     """Time every emit kernel family on this field and method and keep the
     fastest for its (method, element type) in the device's context; later
     calls of isosurface, isosurface_batch and the server on that device use
-    it.  Returns the chosen family (2 lanes along x, 1 lanes along z, 0
-    generic; see KERNEL_FAMILIES)."""
+    it.  Returns the chosen family (3 lanes along x with half-depth bricks,
+    2 lanes along x, 1 lanes along z, 0 generic; see KERNEL_FAMILIES)."""
     method, X, Y, Z = _split_args(method, args, X, Y, Z)
     f, keep, on_dev = _field(sdf)
     p = _params(method, X, Y, Z)

@@ -717,7 +717,7 @@ int msh_set_kernel_family(void* ctx, int32_t method, int32_t dtype, int32_t fami
   Ctx* c = (Ctx*)ctx;
   if (!c) return MSH_ERR_ARGUMENT;
   if (method < 0 || method > 2 || (dtype != K_F32 && dtype != K_F64) || family < -1 ||
-      family > 2)
+      family > 3)
     return fail(c, MSH_ERR_ARGUMENT, "method, dtype or kernel family out of range");
   c->family[method][dtype] = family;
   c->have = false;
@@ -743,7 +743,7 @@ int msh_autotune(void* ctx, const msh_field* f, const msh_params* p, int32_t* fa
   if (e == hipSuccess) e = hipEventCreate(&e1);
   int best = -1;
   float best_ms = 0.f;
-  for (int fam = 2; fam >= 0 && e == hipSuccess; --fam) {
+  for (int fam = 3; fam >= 0 && e == hipSuccess; --fam) {
     Plan q = base;
     q.family = fam;
     VolArgs a = c->vol;

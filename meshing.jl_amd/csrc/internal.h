@@ -34,9 +34,11 @@ struct Plan {
   double epsA; // eps converted to A
   double omeA; // one(T) - eps, computed in promote(T, typeof(eps)), as A
   double a[3], b[3];  // axis endpoints in the axis element type
-  // emit kernel family: 2 lanes along x (k_emit_mc5 / k_emit_h5), 1 lanes
-  // along z (k_emit_mc4 / k_emit_halo), 0 the generic k_emit; -1 the
-  // default order (best first, falling back when a volume does not fit)
+  // emit kernel family: 3 as 2 with half-depth bricks (Marching Cubes with
+  // one element type throughout and Float64 axes; otherwise as 2), 2 lanes
+  // along x (k_emit_mc5 / k_emit_h5), 1 lanes along z (k_emit_mc4 /
+  // k_emit_halo), 0 the generic k_emit; -1 the default order (2, 1, 0,
+  // falling back when a volume does not fit)
   int family = -1;
 };
 
