@@ -117,9 +117,10 @@ int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_p
  * 2); each family falls back to the next one down for volumes it cannot
  * take.
  * msh_autotune counts the given field, times the emit of every family that
- * takes it (three launches each on the context's stream, into a scratch
- * buffer) and records the fastest for the field's (method, element type) in
- * the context, where later msh_count and msh_batch_count calls use it;
+ * takes it (the best of 3 to 50 launches each, about 2 ms, on the
+ * context's stream, into a scratch buffer) and records the fastest for the
+ * field's (method, element type) in the context, where later msh_count and
+ * msh_batch_count calls use it;
  * *family returns the choice.  msh_set_kernel_family sets it directly (-1
  * restores the default).  Results do not depend on the family.  No
  * counterpart in the reference (its tile shapes are the CPU loop). */

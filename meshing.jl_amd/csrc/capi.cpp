@@ -752,8 +752,11 @@ int msh_autotune(void* ctx, const msh_field* f, const msh_params* p, int32_t* fa
     VolArgs probe = a;
     if (variant_family(emit_variant(q, &probe)) != fam) continue;  // not for this volume
     e = launch_emit(q, a, c->stream);  // warm-up
+    // the best of at least 3 launches, and of as many as fill about 2 ms
+    // (short launches vary more)
     float ms = 0.f;
-    for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+    int reps = 3;
+    for (int rep = 0; rep < reps && e == hipSuccess; ++rep) {
       e = hipEventRecord(e0, c->stream);
       if (e == hipSuccess) e = launch_emit(q, a, c->stream);
       if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
@@ -761,6 +764,7 @@ int msh_autotune(void* ctx, const msh_field* f, const msh_params* p, int32_t* fa
       float t = 0.f;
       if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
       if (e == hipSuccess && (rep == 0 || t < ms)) ms = t;
+      if (rep == 0 && t > 0.f) reps = t >= 2.f / 3 ? 3 : (int)(2.f / t) < 50 ? (int)(2.f / t) : 50;
     }
     if (e == hipSuccess && (best < 0 || ms < best_ms)) {
       best = fam;
