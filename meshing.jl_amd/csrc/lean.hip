@@ -269,6 +269,7 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
     // offset of b << 10 | dx,dy,dz of a << 20 | dx,dy,dz of b << 23
     const uint32_t ew = MC_EDGE[threadIdx.x];
     const int ca = ew & 0xF, cb = ew >> 4;
+    static_assert((SY + 1) * RD + DW < (1 << 10), "corner offsets fit 10 bits");
     const uint32_t oa = (uint32_t)((mc_dz(ca) * SY + mc_dy(ca)) * RD + mc_dx(ca) * DW);
     const uint32_t ob = (uint32_t)((mc_dz(cb) * SY + mc_dy(cb)) * RD + mc_dx(cb) * DW);
     sm.le[threadIdx.x] = oa | (ob << 10) |

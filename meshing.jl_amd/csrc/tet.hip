@@ -119,6 +119,7 @@ __device__ __forceinline__ void emit_halo_tile(const VolArgs& a, int64_t t) {
               za = METHOD == MT ? mt_dz(ca) : mc_dz(ca);
     const int xb = METHOD == MT ? mt_dx(cb) : mc_dx(cb), yb = METHOD == MT ? mt_dy(cb) : mc_dy(cb),
               zb = METHOD == MT ? mt_dz(cb) : mc_dz(cb);
+    static_assert((SY + 1) * RD + DW < (1 << 10), "corner offsets fit 10 bits");
     const uint32_t oa = (uint32_t)((za * SY + ya) * RD + xa * DW);
     const uint32_t ob = (uint32_t)((zb * SY + yb) * RD + xb * DW);
     sm.edge[threadIdx.x] = oa | (ob << 10) | ((uint32_t)(xa | (ya << 1) | (za << 2)) << 20) |
