@@ -18,7 +18,10 @@ k_emit_mc5), from HIP events on the stream the library launches on; its
 `traffic` is the HBM bytes per launch from the rocprofv3 FETCH_SIZE and
 WRITE_SIZE summaries kept in profiles/ (null when absent).  `cpu_baseline`
 times the CPU oracle (single thread) on an x-slab of the same field, and
-`cpu_baseline_all_cores` the same sample on up to 16 threads.  A
+`cpu_baseline_all_cores` the same sample on up to 16 threads.  "weld" under
+"float64"/"float32" times the opt-in vertex welding (weld()) of the same
+mesh (algorithmic bytes: vertices read, faces read and written, welded
+vertices written).  A
 1 GiB device-to-device copy is timed at the end as the practical HBM rate
 of the same box (and is the known-size kernel the counter scale factor is
 read from).
@@ -100,7 +103,7 @@ class Runner:
         self.extract()
 
 
-def measure(m, dtype, n, method_name, steps, warmup, kind="value_noise"):
+def measure(m, dtype, n, method_name, steps, warmup, kind="value_noise", with_weld=False):
     import torch
     meth = {"mc": m.MarchingCubes(iso=0), "mt": m.MarchingTetrahedra(iso=0),
             "sn": m.NaiveSurfaceNets(iso=0)}[method_name]
@@ -139,7 +142,24 @@ def measure(m, dtype, n, method_name, steps, warmup, kind="value_noise"):
     res = dict(dtype=dtype, samples=N, nverts=r.nv, nfaces=r.nf, step_s=dt, count_s=t_count,
                family=m.KERNEL_FAMILIES[family], family_id=family,
                emit_s=t_emit, field_bytes=field_bytes, out_bytes=out_bytes,
-               gvox_s=N / dt / 1e9, tris_s=r.nf / dt)
+               gvox_s=N / dt / 1e9, tris_s=r.nf / dt, weld=None)
+    if with_weld:
+        # the opt-in welding step on this mesh: one weld() call (output
+        # allocation, the kernels, the welded-count read-back), best of 3
+        ts = []
+        for _ in range(3):
+            ev[0].record()
+            Vw, Fw = m.weld(r.V, r.F)
+            ev[1].record()
+            torch.cuda.synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]) / 1e3)
+        nu = Vw.shape[0]
+        del Vw, Fw
+        tw = min(ts)
+        wb = r.nv * 3 * osz + 2 * r.nf * 3 * 8 + nu * 3 * osz
+        res["weld"] = dict(ms=round(tw * 1e3, 3), nverts=r.nv, nunique=nu,
+                           algorithmic_bytes=wb, achieved_GB_s=round(wb / tw / 1e9, 1),
+                           frac=round(wb / tw / HBM_PEAK, 4))
     del r, field
     torch.cuda.empty_cache()
     return res
@@ -418,8 +438,8 @@ def main():
     import __graft_entry__ as entry
     m = entry.load_package()
     torch.cuda.init()
-    f64 = measure(m, "float64", a.n, a.method, a.steps, a.warmup)
-    f32 = measure(m, "float32", a.n, a.method, a.steps, a.warmup)
+    f64 = measure(m, "float64", a.n, a.method, a.steps, a.warmup, with_weld=True)
+    f32 = measure(m, "float32", a.n, a.method, a.steps, a.warmup, with_weld=True)
     main_r = f64
     gyroid = {}
     for meth in ("mt", "sn", "mc"):
@@ -457,7 +477,7 @@ def main():
                     nverts=r["nverts"], nfaces=r["nfaces"], algorithmic_bytes=b,
                     autotuned_emit_kernels=r["family"],
                     step_hbm_frac=round(b / r["step_s"] / HBM_PEAK, 4),
-                    roofline=roofline(r))
+                    roofline=roofline(r), weld=r["weld"])
 
     out = {
         "metric": "Gvoxels/s + % HBM roofline (1024^3 MC)",

@@ -127,6 +127,20 @@ int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_p
 int msh_autotune(void* ctx, const msh_field* field, const msh_params* params, int32_t* family);
 int msh_set_kernel_family(void* ctx, int32_t method, int32_t dtype, int32_t family);
 
+/* Vertex welding, the opt-in step after extraction: vertices whose three
+ * coordinates have identical bit patterns merge into their first
+ * appearance, the welded vertices keep first-appearance order, and the
+ * faces (1-based) are renumbered in place.  verts holds nverts x 3
+ * coordinates of `kind` (MSH_FLOAT32 or MSH_FLOAT64); verts, faces and
+ * verts_out are device memory, verts_out with room for nverts x 3; it
+ * receives the *nunique welded vertices.  A face index outside 1..nverts
+ * fails with MSH_ERR_ARGUMENT (the valid ones are renumbered by then).
+ * Returns once *nunique is known.  The reference's Marching Cubes repeats
+ * the vertices that neighbouring cubes share (src/algorithmtypes.jl:17-19)
+ * and leaves merging them to the caller; it has no welding function. */
+int msh_weld(void* ctx, int32_t kind, int64_t nverts, const void* verts, int64_t nfaces,
+             int64_t* faces, void* verts_out, int64_t* nunique);
+
 /* Synthetic fields, generated on the device into a column-major buffer of
  * nx*ny*nz elements of `dtype`.  kind: 0 sphere sqrt(x^2+y^2+z^2)-p0 on
  * [-1,1]^3; 1 torus (R=p0, r=p1) on [-1,1]^3; 2 gyroid on [-p0,p0]^3;

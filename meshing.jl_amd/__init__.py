@@ -34,7 +34,7 @@ from ._capi import (MSH_DEVICE, MSH_FLOAT32, MSH_FLOAT64, MSH_HOST, MSH_INT64, M
 
 __all__ = ["isosurface", "isosurface_batch", "AbstractMeshingAlgorithm", "MarchingCubes",
            "MarchingTetrahedra", "NaiveSurfaceNets", "device_field", "generate_field", "count",
-           "autotune", "set_kernel_family", "KERNEL_FAMILIES", "Server", "FieldSpec"]
+           "autotune", "set_kernel_family", "KERNEL_FAMILIES", "Server", "FieldSpec", "weld"]
 
 # emit kernel families, best first (see msh_autotune in include/meshing_hip.h)
 KERNEL_FAMILIES = {3: "lanes along x, half-depth bricks", 2: "lanes along x",
@@ -393,6 +393,53 @@ def generate_field(kind, shape, dtype="float32", seed=0, p0=None, p1=None, devic
                                                  seed & 0xFFFFFFFFFFFFFFFF, float(p0), float(p1),
                                                  ctypes.c_void_p(buf.data_ptr())))
     return buf.permute(2, 1, 0)
+
+
+def weld(vertices, faces, device=0):
+    """Merge vertices whose three coordinates have identical bit patterns.
+
+    Each group of equal vertices becomes its first appearance, the welded
+    vertices keep first-appearance order, and the 1-based faces are
+    renumbered to them; returns new ``(vertices, faces)`` of the same kind
+    as the arguments (CUDA tensors, or numpy arrays, which travel to the
+    device and back).  Equality is bitwise: -0.0 and 0.0 stay apart, NaNs
+    weld only with the same payload.  The opt-in step after Marching Cubes,
+    whose cubes repeat the vertices they share (src/algorithmtypes.jl:17-19).
+    Two cubes can interpolate a shared edge from opposite ends
+    (src/lut/mc.jl _mc_edge_list), and then its two copies may differ in the
+    last bit and stay apart."""
+    torch = _torch()
+    host = isinstance(vertices, np.ndarray)
+    if host:
+        dev = torch.device("cuda", device)
+        V = torch.from_numpy(np.ascontiguousarray(vertices)).to(dev)
+        F = torch.from_numpy(np.ascontiguousarray(faces, dtype=np.int64)).to(dev)
+    else:
+        if not (_is_device_tensor(vertices) and _is_device_tensor(faces)):
+            raise TypeError("weld takes CUDA tensors or numpy arrays")
+        V = vertices.contiguous()
+        F = faces.to(torch.int64).contiguous().clone()  # renumbered in place
+    if V.ndim != 2 or V.shape[1] != 3 or F.ndim != 2 or F.shape[1] != 3:
+        raise ValueError("vertices and faces must be (n, 3) arrays")
+    if V.dtype not in (torch.float32, torch.float64):
+        raise TypeError("vertex coordinates must be Float32 or Float64")
+    kind = MSH_FLOAT32 if V.dtype == torch.float32 else MSH_FLOAT64
+    out = torch.empty_like(V)
+    n = ctypes.c_int64()
+    ctx = context(V.device.index)
+    L = _capi.lib()
+    with ctx.lock:
+        ctx.check(L.msh_set_stream(
+            ctx.handle, ctypes.c_void_p(torch.cuda.current_stream(V.device).cuda_stream)))
+        ctx.check(L.msh_weld(ctx.handle, kind, V.shape[0],
+                             ctypes.c_void_p(V.data_ptr() if V.numel() else None), F.shape[0],
+                             ctypes.c_void_p(F.data_ptr() if F.numel() else None),
+                             ctypes.c_void_p(out.data_ptr() if out.numel() else None),
+                             ctypes.byref(n)))
+    out = out[:n.value].clone()
+    if host:
+        return out.cpu().numpy(), F.cpu().numpy()
+    return out, F
 
 
 from .serve import FieldSpec, Server  # noqa: E402

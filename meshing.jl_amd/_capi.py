@@ -20,7 +20,7 @@ MSH_HOST, MSH_DEVICE = range(2)
 EXPORTS = ("msh_create", "msh_destroy", "msh_set_stream", "msh_synchronize",
            "msh_last_error", "msh_output_kind", "msh_count", "msh_extract",
            "msh_batch_count", "msh_batch_extract", "msh_autotune", "msh_set_kernel_family",
-           "msh_generate_field")
+           "msh_generate_field", "msh_weld")
 
 
 class MshField(ctypes.Structure):
@@ -94,6 +94,7 @@ def lib():
             L.msh_set_kernel_family.argtypes = [vp, i32, i32, i32]
             L.msh_generate_field.argtypes = [vp, i32, i32, i64, i64, i64, ctypes.c_uint64,
                                              ctypes.c_double, ctypes.c_double, vp]
+            L.msh_weld.argtypes = [vp, i32, i64, vp, i64, vp, vp, ctypes.POINTER(i64)]
             for name in EXPORTS:
                 f = getattr(L, name)
                 if name != "msh_last_error":

@@ -63,6 +63,8 @@ struct Ctx {
   hipEvent_t upload = nullptr;  // last descriptor upload from hbatch
   bool upload_pending = false;
   int64_t* pinned = nullptr;  // 4 x int64 for the totals
+  void* wws = nullptr;        // weld: hash table, scan workspace, indices
+  size_t wws_bytes = 0;
   // emit kernel family per (method, element type), -1 = default order; set
   // by msh_autotune or msh_set_kernel_family
   int family[3][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};
@@ -340,6 +342,7 @@ int msh_destroy(void* ctx) {
   if (c->fstage) (void)hipFree(c->fstage);
   if (c->ostage) (void)hipFree(c->ostage);
   if (c->bws) (void)hipFree(c->bws);
+  if (c->wws) (void)hipFree(c->wws);
   if (c->hbatch) (void)hipHostFree(c->hbatch);
   if (c->upload) (void)hipEventDestroy(c->upload);
   if (c->pinned) (void)hipHostFree(c->pinned);
@@ -777,6 +780,67 @@ int msh_autotune(void* ctx, const msh_field* f, const msh_params* p, int32_t* fa
   c->family[base.method][base.T] = best;
   c->plan.family = best;
   if (family) *family = best;
+  return MSH_OK;
+}
+
+int msh_weld(void* ctx, int32_t kind, int64_t nverts, const void* verts, int64_t nfaces,
+             int64_t* faces, void* verts_out, int64_t* nunique) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  if (kind != K_F32 && kind != K_F64)
+    return fail(c, MSH_ERR_UNSUPPORTED, "vertex coordinates must be Float32 or Float64");
+  if (nverts < 0 || nfaces < 0) return fail(c, MSH_ERR_ARGUMENT, "negative size");
+  if (!nunique) return fail(c, MSH_ERR_ARGUMENT, "null nunique");
+  if ((nverts > 0 && (!verts || !verts_out)) || (nfaces > 0 && !faces))
+    return fail(c, MSH_ERR_ARGUMENT, "null buffer");
+  if (nverts > (int64_t)1 << 40 || nfaces > (int64_t)1 << 40)
+    return fail(c, MSH_ERR_ARGUMENT, "mesh too large");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
+  // workspace: flags | prefixes | first appearances | tiles | sizes | hash table
+  int64_t nslots = 1;
+  while (nslots < nverts + nverts / 4 + 1) nslots <<= 1;
+  const int64_t U = nverts > 0 ? nverts : 1;
+  const int64_t T = (U + 4095) / 4096;
+  const size_t b_flag = align256((size_t)U * 4), b_i64 = align256((size_t)U * 8);
+  const size_t b_tcnt = align256((size_t)(T + 1) * 4), b_toff = align256((size_t)T * 8);
+  const size_t b_tot = 256, b_table = align256((size_t)nslots * 8);
+  const size_t need = b_flag + 2 * b_i64 + b_tcnt + b_toff + b_tot + b_table;
+  int rc = grow(c, &c->wws, &c->wws_bytes, need, "weld workspace allocation");
+  if (rc) return rc;
+  WeldArgs a{};
+  char* q = (char*)c->wws;
+  a.flag = (uint32_t*)q;
+  q += b_flag;
+  a.pos = (int64_t*)q;
+  q += b_i64;
+  a.rep = (int64_t*)q;
+  q += b_i64;
+  a.tilecnt = (uint32_t*)q;
+  q += b_tcnt;
+  a.tileoff = (int64_t*)q;
+  q += b_toff;
+  int64_t* sizes = (int64_t*)q;  // [0] welded count, [1] bad face indices
+  a.total = sizes;
+  a.bad = sizes + 1;
+  q += b_tot;
+  a.table = (int64_t*)q;
+  a.out_kind = kind;
+  a.nverts = nverts;
+  a.nfaces = nfaces;
+  a.verts = verts;
+  a.verts_out = verts_out;
+  a.faces = faces;
+  a.nslots = nslots;
+  hipError_t e = launch_weld(a, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->pinned, sizes, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "weld");
+  const int64_t bad = c->pinned[1];
+  if (bad != 0)
+    return fail(c, MSH_ERR_ARGUMENT,
+                std::to_string(bad) + " face indices outside 1.." + std::to_string(nverts));
+  *nunique = nverts > 0 ? c->pinned[0] : 0;
   return MSH_OK;
 }
 

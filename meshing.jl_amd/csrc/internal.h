@@ -116,6 +116,25 @@ hipError_t launch_sizes_batch(const VolArgs* vols, int n, const Workspace& w, in
 hipError_t launch_emit_batch(const Plan& p, int variant, const VolArgs* vols,
                              const int32_t* emap, int64_t gbase, int64_t nblocks,
                              hipStream_t st);
+// Exact-coordinate vertex welding (weld.hip).
+struct WeldArgs {
+  int out_kind;          // K_F32 or K_F64: element type of the coordinates
+  int64_t nverts, nfaces;
+  const void* verts;     // nverts x 3
+  void* verts_out;       // welded vertices, room for nverts x 3
+  int64_t* faces;        // nfaces x 3, 1-based, renumbered in place
+  int64_t* table;        // hash slots: a power of two >= 1.25 nverts
+  int64_t nslots;
+  int64_t* rep;          // nverts: first appearance, then new index
+  uint32_t* flag;        // nverts: 1 at first appearances
+  int64_t* pos;          // nverts: exclusive prefix of flag
+  uint32_t* tilecnt;     // per 4096 vertices: flags (room rounded up to 8 bytes)
+  int64_t* tileoff;      // per 4096 vertices: exclusive prefix of tilecnt
+  int64_t* total;        // welded count
+  int64_t* bad;          // count of face indices outside 1..nverts
+};
+hipError_t launch_weld(const WeldArgs& a, hipStream_t st);
+
 hipError_t launch_generate(int kind, int dtype, int64_t nx, int64_t ny, int64_t nz,
                            uint64_t seed, double p0, double p1, void* out, hipStream_t st);
 

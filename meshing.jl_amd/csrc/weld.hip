@@ -1,0 +1,237 @@
+// weld.hip -- exact-coordinate vertex welding of a triangle mesh on the
+// device: the opt-in step after Marching Cubes, whose cubes each emit their
+// own copy of a vertex they share (src/algorithmtypes.jl:17-19).
+//
+// Two vertices weld when their three coordinates have identical bit
+// patterns.  The welded vertices keep their order of first appearance and
+// the faces are renumbered (1-based) to them, so the result is what a
+// first-appearance `unique` over the vertex tuples gives on the host.
+//
+//   k_weld_insert   open-addressing hash table over the coordinate bits:
+//                   the first vertex of a key claims a slot (CAS from -1),
+//                   equal keys then lower it to their smallest index
+//                   (atomic min), so every slot ends holding the first
+//                   appearance of its key
+//   k_weld_rep      per vertex: that first appearance; flag = first itself;
+//                   per tile of 4096 vertices: its count of flags
+//   k_weld_tiles    one block: exclusive prefix of the tile counts, total =
+//                   welded count
+//   k_weld_pos      exclusive prefix of the flags = new index of each first
+//                   appearance (reduce-then-scan: no block waits on another)
+//   k_weld_newid    per vertex: new index of its first appearance
+//   k_weld_compact  first appearances to their new slots
+//   k_weld_faces    face indices renumbered in place (out-of-range indices
+//                   are counted and left as they are)
+//
+// Every probe loop ends: the table has more slots than vertices (at least
+// 1.25 times as many), slots never return to empty, and a probe stops at an
+// empty slot or at its own key.
+#include "device.h"
+
+namespace msh {
+namespace {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t h) {
+  // SplitMix64 finaliser (Steele, Lea, Flood, "Fast splittable
+  // pseudorandom number generators", OOPSLA 2014)
+  h ^= h >> 30;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 27;
+  h *= 0x94D049BB133111EBull;
+  h ^= h >> 31;
+  return h;
+}
+
+template <typename W>
+__device__ __forceinline__ uint64_t key_hash(W x, W y, W z) {
+  uint64_t h = mix64((uint64_t)x + 0x9E3779B97F4A7C15ull);
+  h = mix64(h ^ (uint64_t)y);
+  return mix64(h ^ (uint64_t)z);
+}
+
+__global__ void k_fill_i64(int64_t* __restrict__ p, int64_t n, int64_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_weld_insert(const W* __restrict__ v, int64_t nv,
+                                                        unsigned long long* table,
+                                                        uint64_t mask) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const W x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
+  uint64_t h = key_hash(x, y, z) & mask;
+  while (true) {
+    unsigned long long cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == ~0ull) {
+      cur = atomicCAS(&table[h], ~0ull, (unsigned long long)i);
+      if (cur == ~0ull) return;  // claimed
+    }
+    // every index a slot ever holds has the slot's key, so this compare
+    // does not depend on which of them it reads
+    if (v[3 * cur] == x && v[3 * cur + 1] == y && v[3 * cur + 2] == z) {
+      atomicMin(&table[h], (unsigned long long)i);
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+constexpr int kWeldTile = 4096;                 // vertices per prefix tile
+constexpr int kWeldPer = kWeldTile / kBlock;     // per thread in k_weld_pos
+
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_weld_rep(const W* __restrict__ v, int64_t nv,
+                                                     const int64_t* __restrict__ table,
+                                                     uint64_t mask, int64_t* __restrict__ rep,
+                                                     uint32_t* __restrict__ flag,
+                                                     uint32_t* __restrict__ tilecnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool first = false;
+  if (i < nv) {
+    const W x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
+    uint64_t h = key_hash(x, y, z) & mask;
+    int64_t r = i;
+    while (true) {
+      const int64_t cur = table[h];
+      if (cur < 0) break;  // not reached: every key was inserted
+      if (v[3 * cur] == x && v[3 * cur + 1] == y && v[3 * cur + 2] == z) {
+        r = cur;
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+    rep[i] = r;
+    first = r == i;
+    flag[i] = first ? 1u : 0u;
+  }
+  // a wave's 64 vertices lie in one tile (tiles are 64-aligned)
+  const uint64_t b = __ballot(first);
+  if (lane_id() == 0 && b != 0) atomicAdd(&tilecnt[i / kWeldTile], (uint32_t)__popcll(b));
+}
+
+// Exclusive prefix of the flags in three steps: the tile counts (above),
+// their prefix in one block (here), then each tile's own (k_weld_pos).
+__global__ __launch_bounds__(1024) void k_weld_tiles(const uint32_t* __restrict__ tilecnt,
+                                                     int64_t ntiles, int64_t* __restrict__ tileoff,
+                                                     int64_t* __restrict__ total) {
+  __shared__ int64_t s_w[1024 / kWave];
+  const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
+  const int64_t per = (ntiles + 1023) / 1024;
+  const int64_t lo = t * per, hi = lo + per < ntiles ? lo + per : ntiles;
+  int64_t sum = 0;
+  for (int64_t k = lo; k < hi; ++k) sum += tilecnt[k];
+  const int64_t incl = (int64_t)wave_incl_scan64((uint64_t)sum);
+  if (lane == kWave - 1) s_w[wv] = incl;
+  __syncthreads();
+  int64_t before = 0;
+  for (int q = 0; q < wv; ++q) before += s_w[q];
+  int64_t run = before + incl - sum;
+  for (int64_t k = lo; k < hi; ++k) {
+    tileoff[k] = run;
+    run += tilecnt[k];
+  }
+  if (t == 1023) *total = before + incl;
+}
+
+__global__ __launch_bounds__(kBlock) void k_weld_pos(const uint32_t* __restrict__ flag,
+                                                     int64_t nv,
+                                                     const int64_t* __restrict__ tileoff,
+                                                     int64_t* __restrict__ pos) {
+  __shared__ uint32_t s_w[kWaves];
+  const int64_t base = (int64_t)blockIdx.x * kWeldTile + (int64_t)threadIdx.x * kWeldPer;
+  uint32_t f[kWeldPer];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int q = 0; q < kWeldPer; ++q) {
+    f[q] = base + q < nv ? flag[base + q] : 0u;
+    sum += f[q];
+  }
+  const uint32_t incl = wave_scan_dpp(sum);
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_w[wv] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) before += q < wv ? s_w[q] : 0u;
+  int64_t run = tileoff[blockIdx.x] + before + incl - sum;
+#pragma unroll
+  for (int q = 0; q < kWeldPer; ++q) {
+    if (base + q < nv) pos[base + q] = run;
+    run += f[q];
+  }
+}
+
+__global__ void k_weld_newid(int64_t* __restrict__ rep, const int64_t* __restrict__ pos,
+                             int64_t nv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv) rep[i] = pos[rep[i]];
+}
+
+template <typename W>
+__global__ void k_weld_compact(const W* __restrict__ v, int64_t nv,
+                               const int64_t* __restrict__ newid,
+                               const uint32_t* __restrict__ flag, W* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv || !flag[i]) return;
+  const int64_t o = newid[i];
+  out[3 * o] = v[3 * i];
+  out[3 * o + 1] = v[3 * i + 1];
+  out[3 * o + 2] = v[3 * i + 2];
+}
+
+__global__ void k_weld_faces(int64_t* __restrict__ f, int64_t n,
+                             const int64_t* __restrict__ newid, int64_t nv,
+                             unsigned long long* __restrict__ bad) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t j = f[k];
+  if (j >= 1 && j <= nv)
+    f[k] = newid[j - 1] + 1;
+  else
+    atomicAdd(bad, 1ull);
+}
+
+unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <typename W>
+hipError_t weld_t(const WeldArgs& a, hipStream_t st) {
+  const W* v = (const W*)a.verts;
+  hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(kBlock), 0, st, a.bad, (int64_t)1, (int64_t)0);
+  if (a.nverts > 0) {
+    hipLaunchKernelGGL(k_fill_i64, dim3(grid_of(a.nslots)), dim3(kBlock), 0, st, a.table,
+                       a.nslots, (int64_t)-1);
+    hipLaunchKernelGGL((k_weld_insert<W>), dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, v,
+                       a.nverts, (unsigned long long*)a.table, (uint64_t)(a.nslots - 1));
+    const int64_t ntiles = (a.nverts + kWeldTile - 1) / kWeldTile;
+    hipLaunchKernelGGL(k_fill_i64, dim3(grid_of((ntiles + 1) / 2)), dim3(kBlock), 0, st,
+                       (int64_t*)a.tilecnt, (ntiles + 1) / 2, (int64_t)0);
+    hipLaunchKernelGGL((k_weld_rep<W>), dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, v,
+                       a.nverts, (const int64_t*)a.table, (uint64_t)(a.nslots - 1), a.rep,
+                       a.flag, a.tilecnt);
+    hipLaunchKernelGGL(k_weld_tiles, dim3(1), dim3(1024), 0, st, (const uint32_t*)a.tilecnt,
+                       ntiles, a.tileoff, a.total);
+    hipLaunchKernelGGL(k_weld_pos, dim3((unsigned)ntiles), dim3(kBlock), 0, st,
+                       (const uint32_t*)a.flag, a.nverts, (const int64_t*)a.tileoff, a.pos);
+    hipLaunchKernelGGL(k_weld_newid, dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, a.rep,
+                       (const int64_t*)a.pos, a.nverts);
+    hipLaunchKernelGGL((k_weld_compact<W>), dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, v,
+                       a.nverts, (const int64_t*)a.rep, (const uint32_t*)a.flag,
+                       (W*)a.verts_out);
+  }
+  if (a.nfaces > 0)
+    hipLaunchKernelGGL(k_weld_faces, dim3(grid_of(3 * a.nfaces)), dim3(kBlock), 0, st, a.faces,
+                       3 * a.nfaces, (const int64_t*)a.rep, a.nverts,
+                       (unsigned long long*)a.bad);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_weld(const WeldArgs& a, hipStream_t st) {
+  return a.out_kind == K_F32 ? weld_t<uint32_t>(a, st) : weld_t<uint64_t>(a, st);
+}
+
+}  // namespace msh
