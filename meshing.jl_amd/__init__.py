@@ -34,7 +34,8 @@ from ._capi import (MSH_DEVICE, MSH_FLOAT32, MSH_FLOAT64, MSH_HOST, MSH_INT64, M
 
 __all__ = ["isosurface", "isosurface_batch", "AbstractMeshingAlgorithm", "MarchingCubes",
            "MarchingTetrahedra", "NaiveSurfaceNets", "device_field", "generate_field", "count",
-           "autotune", "set_kernel_family", "KERNEL_FAMILIES", "Server", "FieldSpec", "weld"]
+           "autotune", "set_kernel_family", "KERNEL_FAMILIES", "Server", "FieldSpec", "weld",
+           "read_nrrd", "write_nrrd", "NrrdInfo", "read_ply", "write_ply"]
 
 # emit kernel families, best first (see msh_autotune in include/meshing_hip.h)
 KERNEL_FAMILIES = {3: "lanes along x, half-depth bricks", 2: "lanes along x",
@@ -442,4 +443,5 @@ def weld(vertices, faces, device=0):
     return out, F
 
 
+from .io import NrrdInfo, read_nrrd, read_ply, write_nrrd, write_ply  # noqa: E402
 from .serve import FieldSpec, Server  # noqa: E402
