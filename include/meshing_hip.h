@@ -134,8 +134,9 @@ int msh_set_kernel_family(void* ctx, int32_t method, int32_t dtype, int32_t fami
  * coordinates of `kind` (MSH_FLOAT32 or MSH_FLOAT64); verts, faces and
  * verts_out are device memory, verts_out with room for nverts x 3; it
  * receives the *nunique welded vertices.  A face index outside 1..nverts
- * fails with MSH_ERR_ARGUMENT (the valid ones are renumbered by then).
- * Returns once *nunique is known.  The reference's Marching Cubes repeats
+ * fails with MSH_ERR_ARGUMENT (the valid ones are renumbered by then), as
+ * do meshes of more than 3 * 2^30 vertices.  Returns once *nunique is
+ * known.  The reference's Marching Cubes repeats
  * the vertices that neighbouring cubes share (src/algorithmtypes.jl:17-19)
  * and leaves merging them to the caller; it has no welding function. */
 int msh_weld(void* ctx, int32_t kind, int64_t nverts, const void* verts, int64_t nfaces,

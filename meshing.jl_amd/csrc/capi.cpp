@@ -793,8 +793,9 @@ int msh_weld(void* ctx, int32_t kind, int64_t nverts, const void* verts, int64_t
   if (!nunique) return fail(c, MSH_ERR_ARGUMENT, "null nunique");
   if ((nverts > 0 && (!verts || !verts_out)) || (nfaces > 0 && !faces))
     return fail(c, MSH_ERR_ARGUMENT, "null buffer");
-  if (nverts > (int64_t)1 << 40 || nfaces > (int64_t)1 << 40)
-    return fail(c, MSH_ERR_ARGUMENT, "mesh too large");
+  // hash slots are noted as 32-bit numbers: at most 2^32 slots
+  if (nverts > ((int64_t)3 << 30) || nfaces > (int64_t)1 << 40)
+    return fail(c, MSH_ERR_ARGUMENT, "mesh too large (at most 3 * 2^30 vertices)");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
   // workspace: flags | prefixes | first appearances | tiles | sizes | hash table
   int64_t nslots = 1;

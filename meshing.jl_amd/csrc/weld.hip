@@ -10,10 +10,11 @@
 //   k_weld_insert   open-addressing hash table over the coordinate bits:
 //                   the first vertex of a key claims a slot (CAS from -1),
 //                   equal keys then lower it to their smallest index
-//                   (atomic min), so every slot ends holding the first
-//                   appearance of its key
-//   k_weld_rep      per vertex: that first appearance; flag = first itself;
-//                   per tile of 4096 vertices: its count of flags
+//                   (atomic min, only when smaller than what the slot
+//                   showed), so every slot ends holding the first appearance
+//                   of its key; each vertex notes its slot
+//   k_weld_rep      per vertex: its slot's first appearance; flag = first
+//                   itself; per tile of 4096 vertices: its count of flags
 //   k_weld_tiles    one block: exclusive prefix of the tile counts, total =
 //                   welded count
 //   k_weld_pos      exclusive prefix of the flags = new index of each first
@@ -57,7 +58,8 @@ __global__ void k_fill_i64(int64_t* __restrict__ p, int64_t n, int64_t v) {
 template <typename W>
 __global__ __launch_bounds__(kBlock) void k_weld_insert(const W* __restrict__ v, int64_t nv,
                                                         unsigned long long* table,
-                                                        uint64_t mask) {
+                                                        uint64_t mask,
+                                                        uint32_t* __restrict__ slot) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const W x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
@@ -67,42 +69,33 @@ __global__ __launch_bounds__(kBlock) void k_weld_insert(const W* __restrict__ v,
                                                 __HIP_MEMORY_SCOPE_AGENT);
     if (cur == ~0ull) {
       cur = atomicCAS(&table[h], ~0ull, (unsigned long long)i);
-      if (cur == ~0ull) return;  // claimed
+      if (cur == ~0ull) break;  // claimed
     }
     // every index a slot ever holds has the slot's key, so this compare
     // does not depend on which of them it reads
     if (v[3 * cur] == x && v[3 * cur + 1] == y && v[3 * cur + 2] == z) {
-      atomicMin(&table[h], (unsigned long long)i);
-      return;
+      // a slot only ever decreases: above what it showed, i cannot lower it
+      if ((unsigned long long)i < cur) atomicMin(&table[h], (unsigned long long)i);
+      break;
     }
     h = (h + 1) & mask;
   }
+  slot[i] = (uint32_t)h;
 }
 
 constexpr int kWeldTile = 4096;                 // vertices per prefix tile
 constexpr int kWeldPer = kWeldTile / kBlock;     // per thread in k_weld_pos
 
-template <typename W>
-__global__ __launch_bounds__(kBlock) void k_weld_rep(const W* __restrict__ v, int64_t nv,
+__global__ __launch_bounds__(kBlock) void k_weld_rep(int64_t nv,
                                                      const int64_t* __restrict__ table,
-                                                     uint64_t mask, int64_t* __restrict__ rep,
+                                                     const uint32_t* __restrict__ slot,
+                                                     int64_t* __restrict__ rep,
                                                      uint32_t* __restrict__ flag,
                                                      uint32_t* __restrict__ tilecnt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool first = false;
   if (i < nv) {
-    const W x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
-    uint64_t h = key_hash(x, y, z) & mask;
-    int64_t r = i;
-    while (true) {
-      const int64_t cur = table[h];
-      if (cur < 0) break;  // not reached: every key was inserted
-      if (v[3 * cur] == x && v[3 * cur + 1] == y && v[3 * cur + 2] == z) {
-        r = cur;
-        break;
-      }
-      h = (h + 1) & mask;
-    }
+    const int64_t r = table[slot[i]];  // the smallest index of i's key
     rep[i] = r;
     first = r == i;
     flag[i] = first ? 1u : 0u;
@@ -203,14 +196,15 @@ hipError_t weld_t(const WeldArgs& a, hipStream_t st) {
   if (a.nverts > 0) {
     hipLaunchKernelGGL(k_fill_i64, dim3(grid_of(a.nslots)), dim3(kBlock), 0, st, a.table,
                        a.nslots, (int64_t)-1);
+    // the slots live in pos until k_weld_pos overwrites it
+    uint32_t* slot = (uint32_t*)a.pos;
     hipLaunchKernelGGL((k_weld_insert<W>), dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, v,
-                       a.nverts, (unsigned long long*)a.table, (uint64_t)(a.nslots - 1));
+                       a.nverts, (unsigned long long*)a.table, (uint64_t)(a.nslots - 1), slot);
     const int64_t ntiles = (a.nverts + kWeldTile - 1) / kWeldTile;
     hipLaunchKernelGGL(k_fill_i64, dim3(grid_of((ntiles + 1) / 2)), dim3(kBlock), 0, st,
                        (int64_t*)a.tilecnt, (ntiles + 1) / 2, (int64_t)0);
-    hipLaunchKernelGGL((k_weld_rep<W>), dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, v,
-                       a.nverts, (const int64_t*)a.table, (uint64_t)(a.nslots - 1), a.rep,
-                       a.flag, a.tilecnt);
+    hipLaunchKernelGGL(k_weld_rep, dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, a.nverts,
+                       (const int64_t*)a.table, (const uint32_t*)slot, a.rep, a.flag, a.tilecnt);
     hipLaunchKernelGGL(k_weld_tiles, dim3(1), dim3(1024), 0, st, (const uint32_t*)a.tilecnt,
                        ntiles, a.tileoff, a.total);
     hipLaunchKernelGGL(k_weld_pos, dim3((unsigned)ntiles), dim3(kBlock), 0, st,
