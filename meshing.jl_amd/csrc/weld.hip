@@ -7,14 +7,17 @@
 // the faces are renumbered (1-based) to them, so the result is what a
 // first-appearance `unique` over the vertex tuples gives on the host.
 //
-//   k_weld_insert   open-addressing hash table over the coordinate bits:
+//   k_weld_insert   open-addressing hash table over the coordinate bits,
+//                   first within each block's 512 vertices in LDS, then
+//                   over the whole mesh for the block's first appearances:
 //                   the first vertex of a key claims a slot (CAS from -1),
 //                   equal keys then lower it to their smallest index
 //                   (atomic min, only when smaller than what the slot
 //                   showed), so every slot ends holding the first appearance
 //                   of its key; each vertex notes its slot
-//   k_weld_rep      per vertex: its slot's first appearance; flag = first
-//                   itself; per tile of 4096 vertices: its count of flags
+//   k_weld_rep      per vertex: its key's first appearance (one table load
+//                   per local key); flag = first itself; per tile of 4096
+//                   vertices: its count of flags
 //   k_weld_tiles    one block: exclusive prefix of the tile counts, total =
 //                   welded count
 //   k_weld_pos      exclusive prefix of the flags = new index of each first
@@ -24,9 +27,10 @@
 //   k_weld_faces    face indices renumbered in place (out-of-range indices
 //                   are counted and left as they are)
 //
-// Every probe loop ends: the table has more slots than vertices (at least
-// 1.25 times as many), slots never return to empty, and a probe stops at an
-// empty slot or at its own key.
+// Every probe loop ends: each table has more slots than keys (the global
+// one at least 1.25 times the vertices, the local one twice its window),
+// slots never return to empty, and a probe stops at an empty slot or at its
+// own key.
 #include "device.h"
 
 namespace msh {
@@ -55,14 +59,11 @@ __global__ void k_fill_i64(int64_t* __restrict__ p, int64_t n, int64_t v) {
   if (i < n) p[i] = v;
 }
 
+// Global insert of vertex i with key (x, y, z): returns its slot.
 template <typename W>
-__global__ __launch_bounds__(kBlock) void k_weld_insert(const W* __restrict__ v, int64_t nv,
-                                                        unsigned long long* table,
-                                                        uint64_t mask,
-                                                        uint32_t* __restrict__ slot) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  const W x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
+__device__ __forceinline__ uint32_t global_insert(const W* __restrict__ v, int64_t i, W x, W y,
+                                                  W z, unsigned long long* table,
+                                                  uint64_t mask) {
   uint64_t h = key_hash(x, y, z) & mask;
   while (true) {
     unsigned long long cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED,
@@ -80,29 +81,107 @@ __global__ __launch_bounds__(kBlock) void k_weld_insert(const W* __restrict__ v,
     }
     h = (h + 1) & mask;
   }
-  slot[i] = (uint32_t)h;
+  return (uint32_t)h;
+}
+
+constexpr int kWeldLT = 512;           // vertices per block: the local pass's window
+constexpr int kWeldLS = 2 * kWeldLT;   // local hash slots
+
+// Copies of a vertex sit close together in extraction order (the cubes
+// around an edge are neighbours in the same or the next cell row), so each
+// block first welds its 512 consecutive vertices in LDS and sends only
+// the first appearance of each local key to the global table; each vertex
+// notes that local first appearance, whose slot k_weld_rep reads once for
+// all of them.
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_weld_insert(const W* __restrict__ v, int64_t nv,
+                                                        unsigned long long* table,
+                                                        uint64_t mask,
+                                                        uint32_t* __restrict__ slot,
+                                                        uint16_t* __restrict__ lrep) {
+  __shared__ W key[3 * kWeldLT];
+  __shared__ uint32_t lt[kWeldLS];  // local slot -> smallest local index of its key
+  constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+  constexpr int PER = kWeldLT / kBlock;
+  const int64_t base = (int64_t)blockIdx.x * kWeldLT;
+  const int n = nv - base < kWeldLT ? (int)(nv - base) : kWeldLT;
+  for (int k = threadIdx.x; k < kWeldLS; k += kBlock) lt[k] = kEmpty;
+  for (int k = threadIdx.x; k < 3 * n; k += kBlock) key[k] = v[3 * base + k];
+  __syncthreads();
+  uint32_t ls[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int j = threadIdx.x + q * kBlock;
+    ls[q] = 0;
+    if (j >= n) continue;
+    const W x = key[3 * j], y = key[3 * j + 1], z = key[3 * j + 2];
+    uint32_t h = (uint32_t)key_hash(x, y, z) & (kWeldLS - 1);
+    while (true) {
+      uint32_t cur = lt[h];
+      if (cur == kEmpty) {
+        cur = atomicCAS(&lt[h], kEmpty, (uint32_t)j);
+        if (cur == kEmpty) break;
+      }
+      if (key[3 * cur] == x && key[3 * cur + 1] == y && key[3 * cur + 2] == z) {
+        if ((uint32_t)j < cur) atomicMin(&lt[h], (uint32_t)j);
+        break;
+      }
+      h = (h + 1) & (kWeldLS - 1);
+    }
+    ls[q] = h;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int j = threadIdx.x + q * kBlock;
+    if (j >= n) continue;
+    const uint32_t l = lt[ls[q]];
+    lrep[base + j] = (uint16_t)l;
+    if (l == (uint32_t)j)
+      slot[base + j] = global_insert(v, base + j, key[3 * j], key[3 * j + 1], key[3 * j + 2],
+                                     table, mask);
+  }
 }
 
 constexpr int kWeldTile = 4096;                 // vertices per prefix tile
 constexpr int kWeldPer = kWeldTile / kBlock;     // per thread in k_weld_pos
+static_assert(kWeldLT % kBlock == 0 && kWeldTile % kWeldLT == 0 && kWeldLT <= 65536,
+              "windows of whole waves inside one prefix tile, 16-bit local indices");
 
+// Same windows as k_weld_insert: the local first appearances read their
+// key's smallest index from the table, the others take it from them.
 __global__ __launch_bounds__(kBlock) void k_weld_rep(int64_t nv,
                                                      const int64_t* __restrict__ table,
                                                      const uint32_t* __restrict__ slot,
+                                                     const uint16_t* __restrict__ lrep,
                                                      int64_t* __restrict__ rep,
                                                      uint32_t* __restrict__ flag,
                                                      uint32_t* __restrict__ tilecnt) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool first = false;
-  if (i < nv) {
-    const int64_t r = table[slot[i]];  // the smallest index of i's key
-    rep[i] = r;
-    first = r == i;
-    flag[i] = first ? 1u : 0u;
+  __shared__ int64_t res[kWeldLT];
+  constexpr int PER = kWeldLT / kBlock;
+  const int64_t base = (int64_t)blockIdx.x * kWeldLT;
+  const int n = nv - base < kWeldLT ? (int)(nv - base) : kWeldLT;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int j = threadIdx.x + q * kBlock;
+    if (j < n && lrep[base + j] == j) res[j] = table[slot[base + j]];
   }
-  // a wave's 64 vertices lie in one tile (tiles are 64-aligned)
-  const uint64_t b = __ballot(first);
-  if (lane_id() == 0 && b != 0) atomicAdd(&tilecnt[i / kWeldTile], (uint32_t)__popcll(b));
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int j = threadIdx.x + q * kBlock;
+    const int64_t i = base + j;
+    bool first = false;
+    if (j < n) {
+      const int64_t r = res[lrep[i]];  // the smallest index of i's key
+      rep[i] = r;
+      first = r == i;
+      flag[i] = first ? 1u : 0u;
+    }
+    // a wave's 64 vertices here are consecutive and lie in one tile
+    const uint64_t b = __ballot(first);
+    if (lane_id() == 0 && b != 0) atomicAdd(&tilecnt[i / kWeldTile], (uint32_t)__popcll(b));
+  }
 }
 
 // Exclusive prefix of the flags in three steps: the tile counts (above),
@@ -196,15 +275,19 @@ hipError_t weld_t(const WeldArgs& a, hipStream_t st) {
   if (a.nverts > 0) {
     hipLaunchKernelGGL(k_fill_i64, dim3(grid_of(a.nslots)), dim3(kBlock), 0, st, a.table,
                        a.nslots, (int64_t)-1);
-    // the slots live in pos until k_weld_pos overwrites it
+    // slots (4 bytes per vertex) and local first appearances (2 bytes) live
+    // in pos until k_weld_pos overwrites it
     uint32_t* slot = (uint32_t*)a.pos;
-    hipLaunchKernelGGL((k_weld_insert<W>), dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, v,
-                       a.nverts, (unsigned long long*)a.table, (uint64_t)(a.nslots - 1), slot);
+    uint16_t* lrep = (uint16_t*)(slot + a.nverts);
+    const unsigned nwin = (unsigned)((a.nverts + kWeldLT - 1) / kWeldLT);
+    hipLaunchKernelGGL((k_weld_insert<W>), dim3(nwin), dim3(kBlock), 0, st, v, a.nverts,
+                       (unsigned long long*)a.table, (uint64_t)(a.nslots - 1), slot, lrep);
     const int64_t ntiles = (a.nverts + kWeldTile - 1) / kWeldTile;
     hipLaunchKernelGGL(k_fill_i64, dim3(grid_of((ntiles + 1) / 2)), dim3(kBlock), 0, st,
                        (int64_t*)a.tilecnt, (ntiles + 1) / 2, (int64_t)0);
-    hipLaunchKernelGGL(k_weld_rep, dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, a.nverts,
-                       (const int64_t*)a.table, (const uint32_t*)slot, a.rep, a.flag, a.tilecnt);
+    hipLaunchKernelGGL(k_weld_rep, dim3(nwin), dim3(kBlock), 0, st, a.nverts,
+                       (const int64_t*)a.table, (const uint32_t*)slot, (const uint16_t*)lrep,
+                       a.rep, a.flag, a.tilecnt);
     hipLaunchKernelGGL(k_weld_tiles, dim3(1), dim3(1024), 0, st, (const uint32_t*)a.tilecnt,
                        ntiles, a.tileoff, a.total);
     hipLaunchKernelGGL(k_weld_pos, dim3((unsigned)ntiles), dim3(kBlock), 0, st,
