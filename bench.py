@@ -431,8 +431,9 @@ def cpu_baseline(m, n, planes):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "
-                                   "part of it (see DESIGN.md for the x-slab sharding plan)"}))
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
+                                   "of it (one volume would shard by x-slabs with one "
+                                   "all-gather of two counts per rank)"}))
         sys.exit(2)
     import torch
     import __graft_entry__ as entry

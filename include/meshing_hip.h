@@ -132,8 +132,8 @@ int msh_set_kernel_family(void* ctx, int32_t method, int32_t dtype, int32_t fami
  * appearance, the welded vertices keep first-appearance order, and the
  * faces (1-based) are renumbered in place.  verts holds nverts x 3
  * coordinates of `kind` (MSH_FLOAT32 or MSH_FLOAT64); verts, faces and
- * verts_out are device memory, verts_out with room for nverts x 3; it
- * receives the *nunique welded vertices.  A face index outside 1..nverts
+ * verts_out are device memory, verts_out with room for nverts x 3 and
+ * apart from verts; it receives the *nunique welded vertices.  A face index outside 1..nverts
  * fails with MSH_ERR_ARGUMENT (the valid ones are renumbered by then), as
  * do meshes of more than 3 * 2^30 vertices.  Returns once *nunique is
  * known.  The reference's Marching Cubes repeats

@@ -793,6 +793,10 @@ int msh_weld(void* ctx, int32_t kind, int64_t nverts, const void* verts, int64_t
   if (!nunique) return fail(c, MSH_ERR_ARGUMENT, "null nunique");
   if ((nverts > 0 && (!verts || !verts_out)) || (nfaces > 0 && !faces))
     return fail(c, MSH_ERR_ARGUMENT, "null buffer");
+  const size_t vbytes = (size_t)nverts * 3 * (kind == K_F32 ? 4 : 8);
+  if (nverts > 0 && (const char*)verts_out < (const char*)verts + vbytes &&
+      (const char*)verts < (const char*)verts_out + vbytes)
+    return fail(c, MSH_ERR_ARGUMENT, "verts_out overlaps verts");
   // hash slots are noted as 32-bit numbers: at most 2^32 slots
   if (nverts > ((int64_t)3 << 30) || nfaces > (int64_t)1 << 40)
     return fail(c, MSH_ERR_ARGUMENT, "mesh too large (at most 3 * 2^30 vertices)");

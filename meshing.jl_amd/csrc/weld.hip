@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_weld_insert(const W* __restrict__ v,
     const W x = key[3 * j], y = key[3 * j + 1], z = key[3 * j + 2];
     uint32_t h = (uint32_t)key_hash(x, y, z) & (kWeldLS - 1);
     while (true) {
-      uint32_t cur = lt[h];
+      uint32_t cur = __hip_atomic_load(&lt[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (cur == kEmpty) {
         cur = atomicCAS(&lt[h], kEmpty, (uint32_t)j);
         if (cur == kEmpty) break;

@@ -61,6 +61,18 @@ def test_weld_empty_and_small(pkg, oracle):
     with pytest.raises(TypeError):
         pkg.weld(torch.zeros((3, 3), dtype=torch.float16, device="cuda"),
                  torch.tensor([[1, 2, 3]], device="cuda"))
+    # the C ABI refuses an output that overlaps the input
+    import ctypes
+    from importlib import import_module
+    capi = import_module(pkg.__name__ + "._capi")
+    V = torch.zeros((4, 3), dtype=torch.float64, device="cuda")
+    F = torch.tensor([[1, 2, 3]], device="cuda")
+    ctx = capi.context(0)
+    n = ctypes.c_int64()
+    rc = capi.lib().msh_weld(ctx.handle, capi.MSH_FLOAT64, 4, ctypes.c_void_p(V.data_ptr()), 1,
+                             ctypes.c_void_p(F.data_ptr()),
+                             ctypes.c_void_p(V.data_ptr() + 24), ctypes.byref(n))
+    assert rc == capi.MSH_ERR_ARGUMENT
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
