@@ -19,10 +19,8 @@ on one of two HIP streams, and moves on: the next
 batch's classify/count runs on the other stream while this batch's emit is
 still on the GPU.  A completion thread waits for each batch's event,
 resolves its futures and returns its memory to the budget, which lets the
-scheduler admit the requests that arrived meanwhile; a lone request with
-nothing queued behind it and nothing in flight is finished on the
-scheduler thread itself (one thread wake-up less on its latency).  Streams
-keep the default priority.
+scheduler admit the requests that arrived meanwhile.  Streams keep the
+default priority.
 
 There is no counterpart in the reference (Meshing.jl is a library call);
 the extraction itself is the reference's ``isosurface`` (src/isosurface.jl:30-32).
@@ -104,7 +102,6 @@ class Server:
         self.running = True
         self.done_q = collections.deque()
         self.done_cv = threading.Condition()
-        self.completing = False  # the completion thread holds a batch
         # host seconds spent per phase (count + size read-back, output
         # allocation, emit launch, waiting for completion)
         self.stats = dict(batches=0, requests=0, max_batch_seen=0, t_count=0.0, t_alloc=0.0,
@@ -264,18 +261,9 @@ class Server:
             self.stats["batches"] += 1
             self.stats["requests"] += len(batch)
             self.stats["max_batch_seen"] = max(self.stats["max_batch_seen"], len(batch))
-            # a lone request with nothing queued behind it and no batch in
-            # flight is finished here: handing it to the completion thread
-            # would add a thread wake-up to its latency
-            with self.lock:
-                lone = len(batch) == 1 and not self.pending
             with self.done_cv:
-                lone = lone and not self.done_q and not self.completing
-                if not lone:
-                    self.done_q.append((ev, batch))
-                    self.done_cv.notify_all()
-            if lone:
-                self._finish(ev, batch)
+                self.done_q.append((ev, batch))
+                self.done_cv.notify_all()
 
     # -- completion -----------------------------------------------------
     def _complete(self):
@@ -284,36 +272,30 @@ class Server:
                 while not self.done_q:
                     self.done_cv.wait()
                 item = self.done_q.popleft()
-                self.completing = item is not None
             if item is None:
                 return
-            self._finish(*item)
-            with self.done_cv:
-                self.completing = False
-
-    def _finish(self, ev, batch):
-        """Wait for a batch's event, hand out its results, free its memory."""
-        tw = time.perf_counter()
-        ev.synchronize()
-        t = time.perf_counter()
-        self.stats["t_wait"] += t - tw
-        freed = 0
-        for r in batch:
-            r.t_done = t
-            r.extra.pop("field", None)
-            freed += r.reserved
-            r.future.latency = t - r.t_submit
-            cb = r.extra.pop("on_done", None)
-            if cb is None:
-                r.future.set_result(r.result)
-            else:
-                try:
-                    cb(r.result, r.future.latency)
-                    r.future.set_result(None)
-                except BaseException as e:
-                    r.future.set_exception(e)
-            r.result = None
-        batch = None
-        with self.lock:
-            self.in_use -= freed
-            self.lock.notify_all()
+            ev, batch = item
+            tw = time.perf_counter()
+            ev.synchronize()
+            t = time.perf_counter()
+            self.stats["t_wait"] += t - tw
+            freed = 0
+            for r in batch:
+                r.t_done = t
+                r.extra.pop("field", None)
+                freed += r.reserved
+                r.future.latency = t - r.t_submit
+                cb = r.extra.pop("on_done", None)
+                if cb is None:
+                    r.future.set_result(r.result)
+                else:
+                    try:
+                        cb(r.result, r.future.latency)
+                        r.future.set_result(None)
+                    except BaseException as e:
+                        r.future.set_exception(e)
+                r.result = None
+            batch = None
+            with self.lock:
+                self.in_use -= freed
+                self.lock.notify_all()
