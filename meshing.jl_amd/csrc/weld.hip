@@ -66,6 +66,7 @@ __device__ __forceinline__ uint32_t global_insert(const W* __restrict__ v, int64
                                                   uint64_t mask) {
   uint64_t h = key_hash(x, y, z) & mask;
   while (true) {
+    // a plain load first: a CAS on a taken slot measured slower
     unsigned long long cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
     if (cur == ~0ull) {
@@ -208,31 +209,36 @@ __global__ __launch_bounds__(1024) void k_weld_tiles(const uint32_t* __restrict_
   if (t == 1023) *total = before + incl;
 }
 
+// One tile per block in kWeldPer rounds of 256 consecutive vertices, so
+// every load and store of a round is one coalesced stretch.
 __global__ __launch_bounds__(kBlock) void k_weld_pos(const uint32_t* __restrict__ flag,
                                                      int64_t nv,
                                                      const int64_t* __restrict__ tileoff,
                                                      int64_t* __restrict__ pos) {
-  __shared__ uint32_t s_w[kWaves];
-  const int64_t base = (int64_t)blockIdx.x * kWeldTile + (int64_t)threadIdx.x * kWeldPer;
-  uint32_t f[kWeldPer];
-  uint32_t sum = 0;
-#pragma unroll
-  for (int q = 0; q < kWeldPer; ++q) {
-    f[q] = base + q < nv ? flag[base + q] : 0u;
-    sum += f[q];
-  }
-  const uint32_t incl = wave_scan_dpp(sum);
+  __shared__ uint32_t s_w[kWeldPer][kWaves];
   const int lane = lane_id(), wv = threadIdx.x / kWave;
-  if (lane == kWave - 1) s_w[wv] = incl;
-  __syncthreads();
-  uint32_t before = 0;
-#pragma unroll
-  for (int q = 0; q < kWaves; ++q) before += q < wv ? s_w[q] : 0u;
-  int64_t run = tileoff[blockIdx.x] + before + incl - sum;
+  const int64_t base = (int64_t)blockIdx.x * kWeldTile + threadIdx.x;
+  uint32_t f[kWeldPer], incl[kWeldPer];
 #pragma unroll
   for (int q = 0; q < kWeldPer; ++q) {
-    if (base + q < nv) pos[base + q] = run;
-    run += f[q];
+    const int64_t i = base + q * kBlock;
+    f[q] = i < nv ? flag[i] : 0u;
+    incl[q] = wave_scan_dpp(f[q]);
+    if (lane == kWave - 1) s_w[q][wv] = incl[q];
+  }
+  __syncthreads();
+  int64_t run = tileoff[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < kWeldPer; ++q) {
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      before += w < wv ? s_w[q][w] : 0u;
+      all += s_w[q][w];
+    }
+    const int64_t i = base + q * kBlock;
+    if (i < nv) pos[i] = run + before + incl[q] - f[q];
+    run += all;
   }
 }
 
