@@ -125,7 +125,7 @@ struct WeldArgs {
   int64_t* faces;        // nfaces x 3, 1-based, renumbered in place
   int64_t* table;        // hash slots: a power of two >= 1.25 nverts
   int64_t nslots;
-  int64_t* rep;          // nverts: first appearance, then new index
+  int64_t* rep;          // nverts: index of the vertex's first appearance
   uint32_t* flag;        // nverts: 1 at first appearances
   int64_t* pos;          // nverts: exclusive prefix of flag
   uint32_t* tilecnt;     // per 4096 vertices: flags (room rounded up to 8 bytes)

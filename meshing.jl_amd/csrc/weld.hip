@@ -22,9 +22,9 @@
 //                   welded count
 //   k_weld_pos      exclusive prefix of the flags = new index of each first
 //                   appearance (reduce-then-scan: no block waits on another)
-//   k_weld_newid    per vertex: new index of its first appearance
 //   k_weld_compact  first appearances to their new slots
-//   k_weld_faces    face indices renumbered in place (out-of-range indices
+//   k_weld_faces    face indices renumbered in place to the new index of
+//                   their vertex's first appearance (out-of-range indices
 //                   are counted and left as they are)
 //
 // Every probe loop ends: each table has more slots than keys (the global
@@ -242,32 +242,26 @@ __global__ __launch_bounds__(kBlock) void k_weld_pos(const uint32_t* __restrict_
   }
 }
 
-__global__ void k_weld_newid(int64_t* __restrict__ rep, const int64_t* __restrict__ pos,
-                             int64_t nv) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nv) rep[i] = pos[rep[i]];
-}
-
 template <typename W>
 __global__ void k_weld_compact(const W* __restrict__ v, int64_t nv,
-                               const int64_t* __restrict__ newid,
+                               const int64_t* __restrict__ pos,
                                const uint32_t* __restrict__ flag, W* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv || !flag[i]) return;
-  const int64_t o = newid[i];
+  const int64_t o = pos[i];  // a first appearance's new index
   out[3 * o] = v[3 * i];
   out[3 * o + 1] = v[3 * i + 1];
   out[3 * o + 2] = v[3 * i + 2];
 }
 
 __global__ void k_weld_faces(int64_t* __restrict__ f, int64_t n,
-                             const int64_t* __restrict__ newid, int64_t nv,
-                             unsigned long long* __restrict__ bad) {
+                             const int64_t* __restrict__ rep, const int64_t* __restrict__ pos,
+                             int64_t nv, unsigned long long* __restrict__ bad) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int64_t j = f[k];
   if (j >= 1 && j <= nv)
-    f[k] = newid[j - 1] + 1;
+    f[k] = pos[rep[j - 1]] + 1;  // the new index of the vertex's first appearance
   else
     atomicAdd(bad, 1ull);
 }
@@ -298,15 +292,13 @@ hipError_t weld_t(const WeldArgs& a, hipStream_t st) {
                        ntiles, a.tileoff, a.total);
     hipLaunchKernelGGL(k_weld_pos, dim3((unsigned)ntiles), dim3(kBlock), 0, st,
                        (const uint32_t*)a.flag, a.nverts, (const int64_t*)a.tileoff, a.pos);
-    hipLaunchKernelGGL(k_weld_newid, dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, a.rep,
-                       (const int64_t*)a.pos, a.nverts);
     hipLaunchKernelGGL((k_weld_compact<W>), dim3(grid_of(a.nverts)), dim3(kBlock), 0, st, v,
-                       a.nverts, (const int64_t*)a.rep, (const uint32_t*)a.flag,
+                       a.nverts, (const int64_t*)a.pos, (const uint32_t*)a.flag,
                        (W*)a.verts_out);
   }
   if (a.nfaces > 0)
     hipLaunchKernelGGL(k_weld_faces, dim3(grid_of(3 * a.nfaces)), dim3(kBlock), 0, st, a.faces,
-                       3 * a.nfaces, (const int64_t*)a.rep, a.nverts,
+                       3 * a.nfaces, (const int64_t*)a.rep, (const int64_t*)a.pos, a.nverts,
                        (unsigned long long*)a.bad);
   return hipGetLastError();
 }
