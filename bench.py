@@ -21,7 +21,8 @@ times the CPU oracle (single thread) on an x-slab of the same field, and
 `cpu_baseline_all_cores` the same sample on up to 16 threads.  "weld" under
 "float64"/"float32" times the opt-in vertex welding (weld()) of the same
 mesh (algorithmic bytes: vertices read, faces read and written, welded
-vertices written).  A
+vertices written).  "vertex_error_float32_vs_float64" compares the mesh of
+each Float32 field with the mesh of the same samples as Float64.  A
 1 GiB device-to-device copy is timed at the end as the practical HBM rate
 of the same box (and is the known-size kernel the counter scale factor is
 read from).
@@ -365,6 +366,41 @@ def measure_serve(m, nreq, qps_list, seed=11):
     return res
 
 
+def vertex_error(m, kind, n, method_name):
+    """The mesh of a Float32 field against the mesh of the same samples as
+    Float64 (the values convert exactly, so classification and faces are the
+    same and only the arithmetic type differs): max |difference| of the
+    vertex coordinates, that over the cell size and in Float32 ulps of the
+    axis end, and the share of coordinates within one Float32 ulp of the
+    Float64 coordinate (near zero an ulp is tiny, so a few are not)."""
+    import numpy as np
+    import torch
+    meth = {"mc": m.MarchingCubes(iso=0), "mt": m.MarchingTetrahedra(iso=0),
+            "sn": m.NaiveSurfaceNets(iso=0)}[method_name]
+    f32 = m.generate_field(kind, (n, n, n), "float32", seed=1)
+    f64 = f32.permute(2, 1, 0).double().permute(2, 1, 0)  # x fastest, exact values
+    V32, F32 = m.isosurface(f32, meth)
+    V64, F64 = m.isosurface(f64, meth)
+    del f32, f64
+    same = V32.shape == V64.shape and torch.equal(F32, F64)
+    res = dict(method=method_name, field=f"{kind} {n}^3", out_dtype=str(V32.dtype).split(".")[-1],
+               same_faces=bool(same))
+    if same and V32.numel():
+        d = (V32.double() - V64).abs()
+        e = torch.frexp(V64.abs().clamp_min(2.0 ** -126))[1]
+        ulp = torch.ldexp(torch.ones_like(V64), e - 24)
+        half = 2 * np.pi if kind == "gyroid" else 1.0  # the axes run over [-half, half]
+        cell = 2 * half / (n - 1)
+        ulp_range = float(np.spacing(np.float32(half)))
+        res.update(max_abs=float(d.max()), max_over_cell=float(d.max()) / cell,
+                   max_in_f32_ulps_of_axis_end=float(d.max()) / ulp_range,
+                   frac_within_1_f32_ulp_of_coordinate=round(float((d <= ulp).double().mean()), 6))
+        del d, e, ulp
+    del V32, F32, V64, F64
+    torch.cuda.empty_cache()
+    return res
+
+
 def copy_probe():
     """Time a 1 GiB device-to-device tensor copy (2 GiB of HBM traffic)."""
     import torch
@@ -454,6 +490,8 @@ def main():
     batch = measure_batch(m, a.batch_volumes, max(2, min(a.steps, 5))) if a.batch_volumes else None
     serve = (measure_serve(m, a.serve_requests, [float(q) for q in a.serve_qps.split(",")])
              if a.serve_requests else None)
+    verr = [vertex_error(m, "value_noise", a.n, a.method)] + \
+        [vertex_error(m, "gyroid", 256, meth) for meth in ("mt", "sn", "mc")]
     copy = copy_probe()
     cpu, cpu_all = (None, None) if a.no_cpu else cpu_baseline(m, a.n, a.cpu_planes)
     # the emit kernel each autotuned family launches
@@ -504,6 +542,7 @@ def main():
         "gyroid_256_f32": gyroid,
         "batched": batch,
         "serving": serve,
+        "vertex_error_float32_vs_float64": verr,
         "float64": summary(f64),
         "float32": summary(f32),
     }
