@@ -86,3 +86,17 @@ def test_weld_value_noise_256(pkg, oracle, dtype):
     assert torch.equal(Vw.view(torch.uint8), Vw2.view(torch.uint8)) and torch.equal(Fw, Fw2)
     # faces keep their positions
     assert torch.equal(Vw[Fw - 1].view(torch.uint8), V[F - 1].view(torch.uint8))
+
+
+def test_weld_batched_meshes(pkg, oracle):
+    # every mesh of a mixed batch welds to what the definition gives
+    rng = np.random.default_rng(5)
+    fields, methods = [], []
+    for v in range(12):
+        shape = tuple(int(s) for s in rng.integers(8, 40, 3))
+        dt = np.float32 if v % 3 else np.float64
+        fields.append(dev_field(noise(shape, 100 + v, dt)))
+        methods.append([pkg.MarchingCubes(iso=0.0), pkg.NaiveSurfaceNets(),
+                        pkg.MarchingTetrahedra(iso=0.0)][v % 3])
+    for V, F in pkg.isosurface_batch(fields, methods):
+        weld_both(pkg, oracle, V, F)
