@@ -85,6 +85,11 @@ int fail(Ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+int fail(std::string* err, int code, const std::string& msg) {
+  if (err) *err = msg;
+  return code;
+}
+
 int hip_fail(Ctx* c, hipError_t e, const char* what) {
   return fail(c, e == hipErrorOutOfMemory ? MSH_ERR_OUT_OF_MEMORY : MSH_ERR_DEVICE,
               std::string(what) + ": " + hipGetErrorString(e));
@@ -104,9 +109,13 @@ double ceil_to_double(long double x) {
   return d;
 }
 
+}  // namespace
+
+namespace msh {
+
 // Resolve Julia's promotion for this call (src/marching_cubes.jl:31,
 // :100-104; src/marching_tetrahedra.jl:43-54,131) into a Plan.
-int make_plan(Ctx* c, const msh_field* f, const msh_params* p, Plan* out) {
+int make_plan(std::string* c, const msh_field* f, const msh_params* p, Plan* out) {
   if (!f || !p) return fail(c, MSH_ERR_ARGUMENT, "null field or params");
   if (f->dtype != K_F32 && f->dtype != K_F64)
     return fail(c, MSH_ERR_UNSUPPORTED, "field element type must be Float32 or Float64");
@@ -197,7 +206,7 @@ int make_plan(Ctx* c, const msh_field* f, const msh_params* p, Plan* out) {
 
 // batch: the launch has blocks enough from the many volumes, so z rows are
 // split only to keep units within 4096 cells
-Geom make_geom(const msh_field* f, bool batch = false) {
+Geom make_geom(const msh_field* f, bool batch) {
   Geom g{};
   g.nx = (int32_t)f->nx;
   g.ny = (int32_t)f->ny;
@@ -233,7 +242,44 @@ Geom make_geom(const msh_field* f, bool batch = false) {
   return g;
 }
 
-size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+// Workspace of `units` units and `ncoords` coordinates placed at `base`
+// (null: only the size); returns the bytes it takes.
+size_t workspace_layout(int64_t units, int64_t ncoords, void* base, Workspace* w) {
+  const int64_t U = units > 0 ? units : 1;
+  const int64_t nt = scan_tiles(U);
+  const size_t b_cnt = align256((size_t)U * 4);
+  const size_t b_off = align256((size_t)U * 8);
+  const size_t b_status = align256((size_t)(nt + 1) * 8);
+  const size_t b_incl = align256((size_t)nt * 16);
+  const size_t b_tot = 256;
+  const size_t b_coords = align256((size_t)(ncoords + 1) * 8);
+  const size_t need = b_cnt + 2 * b_off + b_status + b_incl + b_tot + b_coords;
+  if (!base || !w) return need;
+  char* p = (char*)base;
+  w->cnt = (uint32_t*)p;
+  p += b_cnt;
+  w->voff = (int64_t*)p;
+  p += b_off;
+  w->foff = (int64_t*)p;
+  p += b_off;
+  w->status = (uint64_t*)p;
+  w->counter = (uint32_t*)(w->status + nt);
+  p += b_status;
+  w->incl = (int64_t*)p;
+  p += b_incl;
+  w->totals = (int64_t*)p;
+  p += b_tot;
+  w->coords = (double*)p;
+  return need;
+}
+
+}  // namespace msh
+
+namespace {
+
+int make_plan(Ctx* c, const msh_field* f, const msh_params* p, Plan* out) {
+  return msh::make_plan(c ? &c->err : nullptr, f, p, out);
+}
 
 int grow(Ctx* c, void** buf, size_t* have, size_t need, const char* what) {
   if (need <= *have) return MSH_OK;
@@ -253,32 +299,10 @@ int grow(Ctx* c, void** buf, size_t* have, size_t need, const char* what) {
 }
 
 int setup_workspace(Ctx* c, int64_t units, int64_t ncoords, Workspace* w) {
-  const int64_t U = units > 0 ? units : 1;
-  const int64_t nt = scan_tiles(U);
-  const size_t b_cnt = align256((size_t)U * 4);
-  const size_t b_off = align256((size_t)U * 8);
-  const size_t b_status = align256((size_t)(nt + 1) * 8);
-  const size_t b_incl = align256((size_t)nt * 16);
-  const size_t b_tot = 256;
-  const size_t b_coords = align256((size_t)(ncoords + 1) * 8);
-  const size_t need = b_cnt + 2 * b_off + b_status + b_incl + b_tot + b_coords;
+  const size_t need = workspace_layout(units, ncoords, nullptr, nullptr);
   int rc = grow(c, &c->ws, &c->ws_bytes, need, "workspace allocation");
   if (rc) return rc;
-  char* p = (char*)c->ws;
-  w->cnt = (uint32_t*)p;
-  p += b_cnt;
-  w->voff = (int64_t*)p;
-  p += b_off;
-  w->foff = (int64_t*)p;
-  p += b_off;
-  w->status = (uint64_t*)p;
-  w->counter = (uint32_t*)(w->status + nt);
-  p += b_status;
-  w->incl = (int64_t*)p;
-  p += b_incl;
-  w->totals = (int64_t*)p;
-  p += b_tot;
-  w->coords = (double*)p;
+  workspace_layout(units, ncoords, c->ws, w);
   return MSH_OK;
 }
 
@@ -377,7 +401,7 @@ int msh_output_kind(const msh_field* f, const msh_params* p, int32_t* kind) {
   msh_field f1 = *f;
   // the kind does not depend on the sizes; avoid the length-1 range check
   f1.nx = f1.ny = f1.nz = 2;
-  int rc = make_plan(nullptr, &f1, p, &pl);
+  int rc = msh::make_plan(nullptr, &f1, p, &pl);
   if (rc) return rc;
   *kind = pl.Out;
   return MSH_OK;

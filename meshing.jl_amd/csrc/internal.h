@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
+#include "../../include/meshing_hip.h"
+
 namespace msh {
 
 enum Kind : int { K_INT64 = 0, K_F32 = 1, K_F64 = 2 };
@@ -86,6 +90,17 @@ constexpr int kScanItems = 8;
 constexpr int kScanTile = 256 * kScanItems;
 
 inline int64_t scan_tiles(int64_t units) { return (units + kScanTile - 1) / kScanTile; }
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Julia's promotion for one call (capi.cpp); on error returns an msh_status
+// and, when err is not null, a message in *err.
+int make_plan(std::string* err, const msh_field* f, const msh_params* p, Plan* out);
+// Units of a volume: z rows cut into segments (batch: split only to keep
+// units within 4096 cells).
+Geom make_geom(const msh_field* f, bool batch = false);
+// Workspace of `units` units and `ncoords` coordinates placed at `base`
+// (null: only the size); returns the bytes it takes.
+size_t workspace_layout(int64_t units, int64_t ncoords, void* base, Workspace* w);
 
 // VolArgs of a single extraction (sz = w.totals, ubase = 0), with the
 // count and emit tilings filled in.

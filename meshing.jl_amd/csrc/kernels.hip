@@ -789,38 +789,20 @@ hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Brick shape (cells along x, y) of the generic emit kernel.  MSH_EMIT_SHAPE
-// picks one of the Marching Cubes shapes (0: 31x2, 1: 31x1, 2: 15x2,
-// 3: 15x1) for tuning; the default is per element type.
-static int emit_shape_choice(int dflt) {
-  const char* e = getenv("MSH_EMIT_SHAPE");
-  if (!e || !*e) return dflt;
-  const int v = atoi(e);
-  return (v >= 0 && v <= 3) ? v : dflt;
-}
-
-static int generic_shape(const Plan& p) {
-  if (p.method == MC) return emit_shape_choice(p.T == K_F32 ? 1 : 3);
-  return p.T == K_F32 ? 1 : 3;
-}
+// Brick shape (cells along x, y) of the generic emit kernel, per element
+// type: 31x1 for Float32, 15x1 for Float64 (of the shapes 0: 31x2, 1: 31x1,
+// 2: 15x2, 3: 15x1).
+static int generic_shape(const Plan& p) { return p.T == K_F32 ? 1 : 3; }
 
 static const int kShapeTX[4] = {31, 31, 15, 15}, kShapeTY[4] = {2, 1, 2, 1};
 
 // Variants: 0..31 = k_emit_mc4 (see mc4_variant); 32 + shape = k_emit;
 // 64 + v = k_emit_halo, tetrahedra and surface nets (see halo_variant);
 // 128 + v = k_emit_mc5 (see mc5_variant); 192 + v = k_emit_h5, tetrahedra
-// and surface nets with lanes along x (see h5_variant).  Comparison runs:
-// MSH_EMIT_V3=1 selects k_emit for every method, MSH_MC_KERNEL=4 k_emit_mc4
-// for cubes, MSH_HALO_KERNEL=4 k_emit_halo for tetrahedra and surface nets.
+// and surface nets with lanes along x (see h5_variant).  The family comes
+// from the plan only (msh_set_kernel_family / msh_autotune).
 int emit_variant(const Plan& p, VolArgs* a) {
-  int fam = p.family < 0 ? 2 : p.family;
-  // comparison runs: environment overrides of the family
-  const char* legacy = getenv("MSH_EMIT_V3");
-  if (legacy && *legacy == '1') fam = 0;
-  const char* mck = getenv("MSH_MC_KERNEL");
-  if (p.method == MC && mck && *mck == '4' && fam > 1) fam = 1;
-  const char* hk = getenv("MSH_HALO_KERNEL");
-  if (p.method != MC && hk && *hk == '4' && fam > 1) fam = 1;
+  const int fam = p.family < 0 ? 2 : p.family;
   if (fam >= 2) {
     const int v = p.method == MC ? mc5_variant(p, a) : h5_variant(p, a);
     if (v >= 0) return (p.method == MC ? 128 : 192) + v;

@@ -574,20 +574,9 @@ hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t
 // Tile shape of the Marching Cubes emit: z chunks of 16 cells and one cell
 // row along y per block (on 1024^3 value noise this beat 32-cell chunks,
 // whose larger LDS brick leaves fewer blocks per CU, and two y rows per
-// block, likewise).  MSH_MC4_SHAPE=1 selects 32-cell chunks for the
-// all-Float32 instantiation with Float64 axes (tuning).
-static int mc4_shape() {
-  static const int sh = [] {
-    const char* e = getenv("MSH_MC4_SHAPE");
-    return (e && atoi(e) == 1) ? 1 : 0;
-  }();
-  return sh;
-}
-
+// block, likewise).
 int mc4_variant(const Plan& p, VolArgs* a) {
-  const bool tunable = p.T == K_F32 && p.M == K_F32 && p.Out == K_F32 && a->cxmask == 0;
-  const int sh = tunable ? mc4_shape() : 0;
-  const int zc = sh ? 32 : 16;
+  const int zc = 16;
   const int64_t esz = p.T == K_F32 ? 4 : 8;
   // the brick's DMA offsets are 32-bit and span ZC+1 planes
   if ((int64_t)(zc + 1) * a->g.nx * a->g.ny * esz >= (1ll << 31)) return -1;
@@ -597,7 +586,7 @@ int mc4_variant(const Plan& p, VolArgs* a) {
   // 16-byte DMA needs 16-byte aligned sample rows
   const bool vec = ((uintptr_t)a->f % 16 == 0) && (((int64_t)a->g.nx * esz) % 16 == 0);
   const int cxclass = a->cxmask == 0 ? 1 : a->cxmask == 7 ? 2 : 0;
-  return (vec ? 0 : 1) | (cxclass << 1) | (sh << 3);
+  return (vec ? 0 : 1) | (cxclass << 1);
 }
 
 template <typename T, typename M, typename Out, int ZC, int CY, int CXM>
@@ -627,11 +616,8 @@ static hipError_t emit_mc4_t(int variant, const VolArgs& a, const VolArgs* vols,
                              const int32_t* emap, int64_t gbase, int64_t nblocks,
                              hipStream_t st) {
   const bool vec = (variant & 1) == 0;
-  const int cxclass = (variant >> 1) & 3, sh = variant >> 3;
+  const int cxclass = (variant >> 1) & 3;
 #define MSH_ARGS vec, a, vols, emap, gbase, nblocks, st
-  if constexpr (sizeof(T) == 4 && std::is_same<T, M>::value && std::is_same<M, Out>::value) {
-    if (cxclass == 1 && sh == 1) return emit_mc4_z<T, M, Out, 32, 1, 0>(MSH_ARGS);
-  }
   if (cxclass == 1) return emit_mc4_z<T, M, Out, 16, 1, 0>(MSH_ARGS);
   if (cxclass == 2) return emit_mc4_z<T, M, Out, 16, 1, 7>(MSH_ARGS);
   return emit_mc4_z<T, M, Out, 16, 1, -1>(MSH_ARGS);

@@ -341,20 +341,11 @@ __global__ __launch_bounds__(kBlock) void k_emit_mc5_b(const VolArgs* __restrict
 template <typename T>
 static constexpr int mc5_zc() { return sizeof(T) == 4 ? 16 : 8; }
 
-// Half-depth bricks (family 3, or MSH_MC5_HALF=1 for the default order):
-// the z chunk of the all-Float32 and all-Float64 instantiations with Float64
-// axes is halved (a smaller LDS brick, more blocks per CU).
-static bool mc5_half() {
-  static const bool h = [] {
-    const char* e = getenv("MSH_MC5_HALF");
-    return e && *e == '1';
-  }();
-  return h;
-}
-
+// Half-depth bricks (family 3): the z chunk of the all-Float32 and
+// all-Float64 instantiations with Float64 axes is halved (a smaller LDS
+// brick, more blocks per CU).
 int mc5_variant(const Plan& p, VolArgs* a) {
-  const bool half = (p.family == 3 || (p.family < 0 && mc5_half())) && p.T == p.M &&
-                    p.M == p.Out && a->cxmask == 0;
+  const bool half = p.family == 3 && p.T == p.M && p.M == p.Out && a->cxmask == 0;
   const int zc = (p.T == K_F32 ? mc5_zc<float>() : mc5_zc<double>()) / (half ? 2 : 1);
   const int64_t esz = p.T == K_F32 ? 4 : 8;
   // the brick's 32-bit DMA offsets reach plane zc, row 4, sample 63

@@ -145,12 +145,17 @@ uint8_t get_cubeindex(const T* v, I iso) {
 }
 inline bool no_triangles(uint8_t c) { return c == 0x00 || c == 0xff; }
 
+// sdf[i, j, k], 1-based.  The data may be an x-slab of a larger array: it
+// then holds planes xoff+1 .. xoff+sx of the array (sx x-samples per row),
+// while nx stays the array's size along x.
 template <class T>
 struct Grid {
   const T* p;
   int64_t nx, ny, nz;
-  T operator()(int64_t i, int64_t j, int64_t k) const {  // 1-based
-    return p[(i - 1) + nx * ((j - 1) + ny * (k - 1))];
+  int64_t xoff = 0, sx = 0;  // sx == 0: the whole array (sx = nx)
+  T operator()(int64_t i, int64_t j, int64_t k) const {
+    const int64_t w = sx ? sx : nx;
+    return p[(i - 1 - xoff) + w * ((j - 1) + ny * (k - 1))];
   }
 };
 
@@ -440,6 +445,9 @@ int out_kind_of(int dtype, const int* kinds, const Scalar& iso, const Scalar* ep
 }
 
 int g_threads = 1;  // > 1: Marching Cubes over x-slabs on that many threads
+// Marching Cubes over the cells with xi in [g_xlo, g_xhi] only (g_xhi < 0:
+// all); set by oracle_isosurface_xslab for one call.
+int64_t g_xlo = 1, g_xhi = -1;
 
 // The multi-threaded form of mc_isosurface (a CPU baseline over all cores,
 // not a restatement): x-slabs on separate threads, concatenated in x order
@@ -474,7 +482,7 @@ void mc_isosurface_threads(const Grid<T>& g, I iso, const Axis (&ax)[3], Result<
 template <class T, class I, class E, class Out>
 void run_method(int method, const Grid<T>& g, I iso, E eps, const Axis (&ax)[3], Result<Out>& r) {
   if (method == 0 && g_threads > 1) mc_isosurface_threads<T, I, Out>(g, iso, ax, r, g_threads);
-  else if (method == 0) mc_isosurface<T, I, Out>(g, iso, ax, r);
+  else if (method == 0) mc_isosurface<T, I, Out>(g, iso, ax, r, g_xlo, g_xhi);
   else if (method == 1) MT<T, I, E, Out>(g, iso, eps, ax, r).run();
   else sn_isosurface<T, I, Out>(g, iso, ax, r);
 }
@@ -548,6 +556,56 @@ int oracle_isosurface(int method, int dtype, const void* sdf, int64_t nx, int64_
     } else {
       throw std::invalid_argument("unsupported field element type");
     }
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// Marching Cubes over the cells xi = xlo..xhi (1-based) of an nx x ny x nz
+// array of which only an x-slab is given: `slab` holds x-planes
+// xoff+1 .. xoff+sx (sx samples per row, x fastest).  Coordinates are those
+// of the whole array (LinRange over nx).  The result is the slice of the
+// whole array's result that these cells emit, with face indices counted
+// from the slice's first vertex (the loop is x-outermost,
+// src/marching_cubes.jl:40, and a cube's faces name only its own vertices).
+int oracle_isosurface_xslab(int dtype, const void* slab, int64_t nx, int64_t ny, int64_t nz,
+                            int64_t xoff, int64_t sx, int64_t xlo, int64_t xhi, int iso_kind,
+                            double iso_f, int64_t iso_i, const int* kinds, const double* lo,
+                            const double* hi, void** verts, int64_t* nverts, int64_t** faces,
+                            int64_t* nfaces, int* out_kind) {
+  try {
+    if (xlo < 1 || xhi > nx - 1 || xlo > xhi || xlo < xoff + 1 || xhi + 1 > xoff + sx ||
+        sx < 1 || xoff + sx > nx)
+      throw std::invalid_argument("x-slab does not hold the requested cells");
+    Axis ax[3] = {make_axis(kinds[0], lo[0], hi[0], nx), make_axis(kinds[1], lo[1], hi[1], ny),
+                  make_axis(kinds[2], lo[2], hi[2], nz)};
+    Scalar iso{iso_kind, iso_f, iso_i}, eps{K_F64, 1e-3, 0};
+    *out_kind = out_kind_of(dtype, kinds, iso, nullptr);
+    const int saved = g_threads;
+    g_threads = 1;
+    g_xlo = xlo;
+    g_xhi = xhi;
+    try {
+      if (dtype == K_F32) {
+        Grid<float> g{(const float*)slab, nx, ny, nz, xoff, sx};
+        run_iso<float>(0, *out_kind, g, iso, eps, ax, verts, nverts, faces, nfaces);
+      } else if (dtype == K_F64) {
+        Grid<double> g{(const double*)slab, nx, ny, nz, xoff, sx};
+        run_iso<double>(0, *out_kind, g, iso, eps, ax, verts, nverts, faces, nfaces);
+      } else {
+        throw std::invalid_argument("unsupported field element type");
+      }
+    } catch (...) {
+      g_threads = saved;
+      g_xlo = 1;
+      g_xhi = -1;
+      throw;
+    }
+    g_threads = saved;
+    g_xlo = 1;
+    g_xhi = -1;
     return 0;
   } catch (const std::exception& e) {
     g_err = e.what();

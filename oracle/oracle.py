@@ -35,6 +35,11 @@ def lib():
                                         ctypes.POINTER(P), ctypes.POINTER(I64),
                                         ctypes.POINTER(I)]
         L.oracle_isosurface.restype = I
+        L.oracle_isosurface_xslab.argtypes = [I, P, I64, I64, I64, I64, I64, I64, I64, I, D,
+                                              I64, P, P, P, ctypes.POINTER(P),
+                                              ctypes.POINTER(I64), ctypes.POINTER(P),
+                                              ctypes.POINTER(I64), ctypes.POINTER(I)]
+        L.oracle_isosurface_xslab.restype = I
         L.oracle_free.argtypes = [P]
         L.oracle_set_threads.argtypes = [I]
         L.oracle_last_error.restype = ctypes.c_char_p
@@ -120,6 +125,59 @@ def isosurface(sdf, method="mc", iso=0.0, eps=1e-3, X=(-1, 1), Y=(-1, 1), Z=(-1,
         L.oracle_free(vp)
         L.oracle_free(fp)
     return V, F
+
+
+def _collect(vp, nv, fp, nf, ok):
+    odt = np.float32 if ok.value == K_F32 else np.float64
+    L = lib()
+    try:
+        V = np.ctypeslib.as_array((ctypes.c_char * (nv.value * 3 * np.dtype(odt).itemsize))
+                                  .from_address(vp.value)).view(odt).reshape(-1, 3).copy() \
+            if nv.value else np.zeros((0, 3), odt)
+        F = np.ctypeslib.as_array((ctypes.c_int64 * (nf.value * 3)).from_address(fp.value)) \
+            .reshape(-1, 3).copy() if nf.value else np.zeros((0, 3), np.int64)
+    finally:
+        L.oracle_free(vp)
+        L.oracle_free(fp)
+    return V, F
+
+
+def isosurface_xslab(slab, nx, xoff, xlo, xhi, iso=0.0, X=(-1, 1), Y=(-1, 1), Z=(-1, 1)):
+    """Marching Cubes over the cells with x index xlo..xhi (1-based) of an
+    array that is nx samples long along x, given only its x-planes
+    xoff+1 .. xoff+slab.shape[0] (``slab`` indexed [x, y, z]).  Returns the
+    slice of the whole array's mesh that these cells emit, with faces
+    counted from the slice's first vertex: the reference's loop runs x
+    outermost (src/marching_cubes.jl:40) and a cube's faces name only its
+    own vertices, so the whole mesh is these slices concatenated with
+    shifted faces."""
+    slab = np.asarray(slab)
+    if slab.dtype == np.float32:
+        dtype = K_F32
+    elif slab.dtype == np.float64:
+        dtype = K_F64
+    else:
+        raise TypeError(f"unsupported field dtype {slab.dtype}")
+    a = np.asfortranarray(slab)
+    sx, ny, nz = a.shape
+    kinds, lo, hi = [], [], []
+    for R in (X, Y, Z):
+        f, l = _endpoints(R)
+        kinds.append(kind_of(f))
+        lo.append(float(f))
+        hi.append(float(l))
+    ik = kind_of(iso)
+    vp, fp = ctypes.c_void_p(), ctypes.c_void_p()
+    nv, nf, ok = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+    L = lib()
+    rc = L.oracle_isosurface_xslab(dtype, a.ctypes.data, nx, ny, nz, xoff, sx, xlo, xhi, ik,
+                                   float(iso), int(iso) if ik == K_INT64 else 0,
+                                   (ctypes.c_int * 3)(*kinds), (ctypes.c_double * 3)(*lo),
+                                   (ctypes.c_double * 3)(*hi), ctypes.byref(vp), ctypes.byref(nv),
+                                   ctypes.byref(fp), ctypes.byref(nf), ctypes.byref(ok))
+    if rc != 0:
+        raise ValueError(L.oracle_last_error().decode())
+    return _collect(vp, nv, fp, nf, ok)
 
 
 def linrange(first, last, n):
