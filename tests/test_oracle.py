@@ -84,6 +84,17 @@ def test_linrange_matches_julia_documentation(oracle):
     assert oracle.linrange(-1, 1, 3).tolist() == [-1.0, 0.0, 1.0]
 
 
+def test_linrange_docstring_rejects_other_roundings():
+    # The docstring values above tell lerpi's (1-t)*a + t*b apart from the
+    # other usual formulas: a + t*(b-a) and a + i*step (numpy's linspace)
+    # both give 0.0 for the second value, not -1.3877787807814457e-17.
+    a, b, n = -0.1, 0.3, 5
+    t = np.arange(n) / (n - 1)
+    assert ((1 - t) * a + t * b)[1] == -1.3877787807814457e-17
+    assert (a + t * (b - a))[1] == 0.0
+    assert np.linspace(a, b, n)[1] == 0.0
+
+
 def test_linrange_endpoints_and_float32(oracle):
     for n in (2, 3, 7, 128, 1000):
         r = oracle.linrange(-1, 1, n)
@@ -251,13 +262,169 @@ def test_mc_faces_stay_in_their_cube(oracle):
     assert F.max() == len(V)
 
 
-def test_sn_vertices_inside_cells(oracle):
-    rng = np.random.default_rng(4)
-    f = rng.standard_normal((12, 11, 10))
-    V, F = oracle.isosurface(f, "sn", iso=0, X=(0, 11), Y=(0, 10), Z=(0, 9))
-    frac = V - np.floor(V)
-    assert np.all(V >= 0) and np.all(V[:, 0] <= 11) and np.all(V[:, 1] <= 10)
-    assert len(F) % 2 == 0
+# ---- closed forms of the counts and the numbering ---------------------------
+
+def below(f, iso):
+    """The classification bit of every sample: `v < iso`, NaN gives false
+    (src/common.jl:10-20)."""
+    with np.errstate(invalid="ignore"):
+        return np.asarray(f) < iso
+
+
+def kuhn_crossing_edges(b):
+    """Crossing edges of the 6-tetrahedron split of every voxel, on the
+    global lattice: the 3 axis edges, the 3 face diagonals that run from a
+    face's low corner to its high corner, and the body diagonal (the split
+    of src/lut/mt.jl shares corners (0,0,0) and (1,1,1) in every
+    tetrahedron).  Each edge is counted once."""
+    n = 0
+    for d in [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1)]:
+        lo = b[:b.shape[0] - d[0], :b.shape[1] - d[1], :b.shape[2] - d[2]]
+        hi = b[d[0]:, d[1]:, d[2]:]
+        n += int((lo != hi).sum())
+    return n
+
+
+def kuhn_triangles(b):
+    """Triangles of the 6 tetrahedra (0,0,0), e_a, e_a+e_b, (1,1,1) of every
+    voxel: one when 1 or 3 of the 4 corners are below, two when 2 are."""
+    c0 = b[:-1, :-1, :-1].astype(np.int8)
+    c7 = b[1:, 1:, 1:].astype(np.int8)
+    unit = [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    nx, ny, nz = b.shape
+
+    def at(d):
+        return b[d[0]:nx - 1 + d[0], d[1]:ny - 1 + d[1], d[2]:nz - 1 + d[2]].astype(np.int8)
+
+    total = 0
+    for a, c in itertools.permutations(range(3), 2):
+        ea = unit[a]
+        eab = tuple(p + q for p, q in zip(unit[a], unit[c]))
+        k = c0 + at(ea) + at(eab) + c7
+        total += int(((k == 1) | (k == 3)).sum()) + 2 * int((k == 2).sum())
+    return total
+
+
+def first_appearance_order(F):
+    """Vertex indices in the order they first appear in the flattened faces."""
+    flat = F.reshape(-1)
+    _, first = np.unique(flat, return_index=True)
+    return flat[np.sort(first)]
+
+
+MT_FIELDS = [
+    ("noise", lambda: np.random.default_rng(11).standard_normal((9, 8, 7)), 0.0),
+    ("noise-iso", lambda: np.random.default_rng(12).standard_normal((6, 10, 5)).astype(np.float32),
+     0.3),
+    ("sphere", lambda: sphere(17, 0.55), 0.0),
+    ("torus", lambda: torus(15), 0.0),
+    ("thin", lambda: np.random.default_rng(13).standard_normal((2, 7, 9)), -0.2),
+]
+
+
+@pytest.mark.parametrize("name,make,iso", MT_FIELDS, ids=[m[0] for m in MT_FIELDS])
+def test_mt_counts_closed_form(oracle, name, make, iso):
+    # getVertId creates one vertex per distinct crossing tetrahedron edge
+    # (vertId names an edge the same from every voxel,
+    # src/marching_tetrahedra.jl:29-32,67-84) and tetTri gives 1 or 2
+    # triangles per cut tetrahedron.  A slip in the Dict key (two edges one
+    # key, or one edge two keys) changes V.
+    f = make()
+    V, F = oracle.isosurface(f, "mt", iso=iso)
+    b = below(f, iso)
+    assert len(V) == kuhn_crossing_edges(b) > 0
+    assert len(F) == kuhn_triangles(b)
+
+
+@pytest.mark.parametrize("name,make,iso", MT_FIELDS, ids=[m[0] for m in MT_FIELDS])
+def test_mt_numbering_is_first_request(oracle, name, make, iso):
+    # A vertex is numbered length(vtsAry)+1 when it is first requested, and
+    # every request is an argument of a face pushed at once, left to right
+    # (src/marching_tetrahedra.jl:67-84,117-125).  So the vertex indices, in
+    # the order they first appear in the flattened faces, are exactly 1..V.
+    # Numbering a voxel's vertices before its faces, or evaluating a face's
+    # arguments in another order, breaks this.
+    V, F = oracle.isosurface(make(), "mt", iso=iso)
+    assert np.array_equal(first_appearance_order(F), np.arange(1, len(V) + 1))
+    # the check has teeth: the same faces with reversed arguments fail it
+    assert not np.array_equal(first_appearance_order(F[:, ::-1].copy()),
+                              np.arange(1, len(V) + 1))
+
+
+def test_mc_numbering_is_first_appearance(oracle):
+    # process_mc_voxel! pushes a cube's vertices in _mc_verts order and its
+    # first face is (f+3, f+2, f+1) (src/marching_cubes.jl:69-90), so the
+    # vertices appear in order cube by cube: reading each face backwards,
+    # first appearances are 1..V
+    f = np.random.default_rng(14).standard_normal((8, 9, 7))
+    V, F = oracle.isosurface(f, "mc", iso=0.1)
+    assert np.array_equal(first_appearance_order(F[:, ::-1].copy()), np.arange(1, len(V) + 1))
+
+
+SN_FIELDS = [
+    ("noise", lambda: np.random.default_rng(4).standard_normal((12, 11, 10)), 0.0),
+    ("noise-f32", lambda: np.random.default_rng(5).standard_normal((7, 9, 8)).astype(np.float32),
+     0.25),
+    ("sphere", lambda: sphere(19, 0.6), 0.0),
+    ("thin", lambda: np.random.default_rng(6).standard_normal((2, 6, 5)), 0.0),
+]
+
+
+@pytest.mark.parametrize("name,make,iso", SN_FIELDS, ids=[m[0] for m in SN_FIELDS])
+def test_sn_spec_invariants(oracle, name, make, iso):
+    # The surface-nets spec (one vertex per active cell, numbered in the
+    # reference's cell order, placed in its cell; one quad per crossing grid
+    # edge whose four cells exist, emitted by the cell at the edge's low
+    # corner, as two triangles that share that cell's vertex), checked
+    # against counts and positions computed here from the samples alone.
+    f = make()
+    nx, ny, nz = f.shape
+    X, Y, Z = (0.0, float(nx - 1)), (-2.0, 2.0 * (ny - 1) - 2.0), (1.0, 1.0 + 0.5 * (nz - 1))
+    V, F = oracle.isosurface(f, "sn", iso=iso, X=X, Y=Y, Z=Z)
+    b = below(f, iso)
+    corners = [b[dx:nx - 1 + dx, dy:ny - 1 + dy, dz:nz - 1 + dz]
+               for dx, dy, dz in MC_CORNERS]
+    s = sum(c.astype(np.int8) for c in corners)
+    active = (s > 0) & (s < 8)
+    cells = np.argwhere(active)  # row-major over (x, y, z): z fastest, the reference's order
+    assert len(V) == len(cells)
+    # each vertex lies in its cell (up to the rounding of a mean)
+    lo = np.array([X[0], Y[0], Z[0]]) + cells * np.array(
+        [(X[1] - X[0]) / (nx - 1), (Y[1] - Y[0]) / (ny - 1), (Z[1] - Z[0]) / (nz - 1)])
+    hi = lo + np.array([(X[1] - X[0]) / (nx - 1), (Y[1] - Y[0]) / (ny - 1),
+                        (Z[1] - Z[0]) / (nz - 1)])
+    tol = 1e-5 if V.dtype == np.float32 else 1e-12
+    assert np.all(V >= lo - tol) and np.all(V <= hi + tol)
+    # quads: crossing edges with all four cells present
+    q = int((b[:-1, 1:-1, 1:-1] != b[1:, 1:-1, 1:-1]).sum()
+            + (b[1:-1, :-1, 1:-1] != b[1:-1, 1:, 1:-1]).sum()
+            + (b[1:-1, 1:-1, :-1] != b[1:-1, 1:-1, 1:]).sum())
+    assert len(F) == 2 * q
+    if q:
+        # every face leads with the emitting cell's vertex, the largest index
+        # of the face, in cell order; the two triangles of a quad share it
+        # and the quad's diagonal
+        assert np.all(F[:, 0] > F[:, 1]) and np.all(F[:, 0] > F[:, 2])
+        assert np.all(np.diff(F[:, 0]) >= 0)
+        assert np.array_equal(F[0::2, 0], F[1::2, 0]) and np.array_equal(F[0::2, 2], F[1::2, 1])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_xslab_oracle_concatenates_to_the_whole(oracle, dtype):
+    # the x-slab form of the Marching Cubes oracle, used to check meshes
+    # too large for the whole-field oracle: slabs at the start, middle and
+    # end concatenate (faces shifted) to the whole mesh
+    f = np.random.default_rng(21).standard_normal((13, 9, 8)).astype(dtype)
+    nx = f.shape[0]
+    V, F = oracle.isosurface(f, "mc", iso=0.05)
+    parts, nv = [], 0
+    for xlo, xhi in ((1, 3), (4, 4), (5, 9), (10, 12)):
+        slab = f[xlo - 1:xhi + 1]
+        Vs, Fs = oracle.isosurface_xslab(slab, nx, xlo - 1, xlo, xhi, iso=0.05)
+        parts.append((Vs, Fs + nv))
+        nv += len(Vs)
+    assert np.array_equal(np.concatenate([p[0] for p in parts]), V)
+    assert np.array_equal(np.concatenate([p[1] for p in parts]), F)
 
 
 def test_empty_and_degenerate(oracle):
