@@ -55,7 +55,7 @@ def parse():
                     help="volumes of the batched-vs-plain measurement (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the serving measurement (0 skips it)")
-    ap.add_argument("--serve-qps", default="100,200,400,800,0",
+    ap.add_argument("--serve-qps", default="200,400,800,1200,1600,2000,2400,0",
                     help="offered request rates of the serving sweep (0: all at once)")
     return ap.parse_args()
 

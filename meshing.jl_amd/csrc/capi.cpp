@@ -397,6 +397,7 @@ const char* msh_last_error(const void* ctx) {
 }
 
 int msh_output_kind(const msh_field* f, const msh_params* p, int32_t* kind) {
+  if (!f || !p || !kind) return MSH_ERR_ARGUMENT;
   Plan pl;
   msh_field f1 = *f;
   // the kind does not depend on the sizes; avoid the length-1 range check

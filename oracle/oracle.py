@@ -25,7 +25,9 @@ _LIB = None
 def lib():
     global _LIB
     if _LIB is None:
-        path = _HERE / "liboracle.so"
+        # MSH_ORACLE_LIB names another build of cpu_ref.cpp (the sanitizer
+        # build, `make -C oracle sanitize`); test infrastructure only
+        path = Path(os.environ.get("MSH_ORACLE_LIB", _HERE / "liboracle.so"))
         if not path.exists():
             raise RuntimeError(f"{path} is not built: run `make -C {_HERE}`")
         L = ctypes.CDLL(str(path))
