@@ -24,10 +24,14 @@
 // Naive Surface Nets has no code in v0.7.0; its definition here is this
 // project's own (see sn_isosurface below).
 //
-// Parity unpinned (no Julia run is possible here): the interior rounding of
-// LinRange (only its endpoints, symmetry and the exact values of Julia's
-// LinRange docstring are checked), and the noisy-sphere counts of
-// test/runtests.jl:152-172, which depend on Julia's MersenneTwister stream.
+// Pinning: LinRange's Float64 formula by the exact values of Julia's
+// LinRange docstring, which (1-t)*a + t*b reproduces and a + t*(b-a) and
+// a + i*step do not; MT's vertex numbering by first appearance in the
+// flattened faces and its counts by closed forms over the tetrahedron
+// split; the surface-nets spec by its counting and placement invariants.
+// Parity unpinned (no Julia run is possible here): LinRange values beyond
+// the docstring's, and the noisy-sphere counts of test/runtests.jl:152-172,
+// which depend on Julia's MersenneTwister stream.
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
