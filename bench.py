@@ -422,12 +422,105 @@ def copy_probe():
                 GBps=round(2 * 1024 ** 3 / best / 1e9, 1))
 
 
-def traffic_of(method, n, dtype):
-    """HBM bytes per emit launch from the committed rocprofv3 counter summary."""
-    tfile = ROOT / "profiles" / f"traffic_{method}_value_noise_{n}.json"
-    if not tfile.exists():
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_emit_kernels.json"
+
+
+def traffic_of(workload, family_id):
+    """HBM bytes per emit launch of `workload` from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
+    known size), when the summary was taken on the same emit kernel family
+    as the one timed here; otherwise None."""
+    if not TRAFFIC_FILE.exists():
         return None
-    return json.loads(tfile.read_text()).get(dtype, {}).get("bytes_per_launch")
+    e = json.loads(TRAFFIC_FILE.read_text()).get("workloads", {}).get(workload)
+    if not e or e.get("family_id") != family_id:
+        return None
+    return e["bytes_per_launch"]
+
+
+# the emit kernel an autotuned family launches, per method: (name, template
+# prefix after the method parameter); MC's brick depth per element type and
+# family (family 3 halves it)
+_EMIT_NAMES = {"mc": {3: "k_emit_mc5", 2: "k_emit_mc5", 1: "k_emit_mc4", 0: "k_emit"},
+               "mt": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"},
+               "sn": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"}}
+_MC5_ZC = {("float32", 2): 16, ("float32", 3): 8, ("float64", 2): 8, ("float64", 3): 4}
+
+
+def _emit_pattern(method, dtype, family_id):
+    """A regular expression matching the demangled name of the emit kernel
+    that `family_id` launches for this (method, field type)."""
+    import re
+    name = _EMIT_NAMES[method][family_id]
+    t = "float" if dtype == "float32" else "double"
+    if name == "k_emit_mc5":
+        return re.compile(rf"k_emit_mc5<{t}, [a-z]+, [a-z]+, {_MC5_ZC[(dtype, family_id)]},")
+    mid = {"mc": 0, "mt": 1, "sn": 2}[method]
+    if name == "k_emit":
+        return re.compile(rf"k_emit<{mid}, {t},")
+    return re.compile(rf"{name}<{mid}, {t},")
+
+
+def summarize_counters(fetch_dir, write_dir, line_file, out_file=TRAFFIC_FILE, copy_bytes=1 << 30):
+    """Turn the counter runs of this benchmark (rocprofv3 --pmc FETCH_SIZE,
+    then --pmc WRITE_SIZE, each over `bench.py --steps 1 --warmup 0 --no-cpu
+    --batch-volumes 0 --serve-requests 0`) into the traffic summary that the
+    roofline reads.  `line_file` holds the JSON line the counter run printed:
+    it says which emit family each workload timed, which picks the kernel
+    instantiation.  Counter values are KiB per dispatch; each counter is
+    scaled by what it reads for the closing device copies of `copy_bytes`
+    (the last four __amd_rocclr_copyBuffer dispatches)."""
+    import csv
+    line = json.loads(Path(line_file).read_text().strip().splitlines()[-1])
+
+    def rows(d, counter):
+        path = next(Path(d).rglob("*counter_collection.csv"))
+        with open(path, newline="") as fh:
+            return [r for r in csv.DictReader(fh) if r["Counter_Name"] == counter]
+
+    fetch, write = rows(fetch_dir, "FETCH_SIZE"), rows(write_dir, "WRITE_SIZE")
+    scale = {}
+    for counter, rs in (("FETCH_SIZE", fetch), ("WRITE_SIZE", write)):
+        cp = [float(r["Counter_Value"]) for r in rs if "copyBuffer" in r["Kernel_Name"]][-4:]
+        scale[counter] = (copy_bytes / 1024.0) / (sum(cp) / len(cp))
+    loads = {"mc_value_noise_1024_float64": ("mc", "float64", line["float64"], max),
+             "mc_value_noise_1024_float32": ("mc", "float32", line["float32"], max)}
+    for meth, g in line.get("gyroid_256_f32", {}).items():
+        loads[f"{meth}_gyroid_256_float32"] = (meth, "float32", g, min)
+    out = {"note": "HBM bytes per emit launch (FETCH_SIZE and WRITE_SIZE, KiB per dispatch, "
+                   "averaged over the dispatches of the instantiation, times the scale a "
+                   "1 GiB device copy gives each counter in the same run)",
+           "scale": {k: round(v, 4) for k, v in scale.items()}, "workloads": {}}
+    for key, (meth, dtype, r, pick) in loads.items():
+        fam = r["family_id"]
+        pat = _emit_pattern(meth, dtype, fam)
+        per = {}
+        for counter, rs in (("FETCH_SIZE", fetch), ("WRITE_SIZE", write)):
+            grids = {}
+            for x in rs:
+                if pat.search(x["Kernel_Name"]):
+                    grids.setdefault((x["Kernel_Name"], int(x["Grid_Size"])), []).append(
+                        float(x["Counter_Value"]))
+            if not grids:
+                break
+            # the 1024^3 launches have the largest grid of their instantiation,
+            # the 256^3 ones the smallest
+            kname, grid = pick(grids, key=lambda k: k[1])
+            vals = grids[(kname, grid)]
+            per[counter] = dict(kernel=kname, grid=grid, dispatches=len(vals),
+                                kib=sum(vals) / len(vals))
+        if len(per) < 2:
+            continue
+        fb = per["FETCH_SIZE"]["kib"] * 1024 * scale["FETCH_SIZE"]
+        wb = per["WRITE_SIZE"]["kib"] * 1024 * scale["WRITE_SIZE"]
+        alg = r.get("algorithmic_bytes") or r["roofline"]["algorithmic_bytes_per_launch"]
+        out["workloads"][key] = dict(
+            kernel=per["FETCH_SIZE"]["kernel"], grid=per["FETCH_SIZE"]["grid"],
+            family_id=fam, dispatches=per["FETCH_SIZE"]["dispatches"],
+            fetch_bytes=int(fb), write_bytes=int(wb), bytes_per_launch=int(fb + wb),
+            algorithmic_bytes=alg, traffic_over_algorithmic=round((fb + wb) / alg, 3))
+    Path(out_file).write_text(json.dumps(out, indent=1) + "\n")
+    return out
 
 
 def cpu_baseline(m, n, planes):
@@ -465,6 +558,10 @@ def cpu_baseline(m, n, planes):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-counters":
+        # maintenance: bench.py --summarize-counters FETCH_DIR WRITE_DIR LINE_FILE
+        print(json.dumps(summarize_counters(*sys.argv[2:5]), indent=1))
+        return
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
@@ -475,6 +572,15 @@ def main():
     import __graft_entry__ as entry
     m = entry.load_package()
     torch.cuda.init()
+    def roofline(r, meth, workload):
+        b = r["field_bytes"] + r["out_bytes"]
+        ach = b / r["emit_s"]
+        return {"bound": "hbm", "achieved": round(ach / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4),
+                "traffic": traffic_of(workload, r["family_id"]),
+                "kernel": _EMIT_NAMES[meth].get(r["family_id"], "k_emit"),
+                "algorithmic_bytes_per_launch": b, "avg_launch_ms": round(r["emit_s"] * 1e3, 4)}
+
     f64 = measure(m, "float64", a.n, a.method, a.steps, a.warmup, with_weld=True)
     f32 = measure(m, "float32", a.n, a.method, a.steps, a.warmup, with_weld=True)
     main_r = f64
@@ -482,11 +588,12 @@ def main():
     for meth in ("mt", "sn", "mc"):
         r = measure(m, "float32", 256, meth, a.steps, a.warmup, kind="gyroid")
         gyroid[meth] = dict(gvoxels_s=round(r["gvox_s"], 3), tris_s=round(r["tris_s"], 1),
-                            autotuned_emit_kernels=r["family"],
+                            autotuned_emit_kernels=r["family"], family_id=r["family_id"],
                             ms_per_step=round(r["step_s"] * 1e3, 4),
                             count_ms=round(r["count_s"] * 1e3, 4),
                             emit_ms=round(r["emit_s"] * 1e3, 4),
-                            nverts=r["nverts"], nfaces=r["nfaces"])
+                            nverts=r["nverts"], nfaces=r["nfaces"],
+                            roofline=roofline(r, meth, f"{meth}_gyroid_256_float32"))
     batch = measure_batch(m, a.batch_volumes, max(2, min(a.steps, 5))) if a.batch_volumes else None
     serve = (measure_serve(m, a.serve_requests, [float(q) for q in a.serve_qps.split(",")])
              if a.serve_requests else None)
@@ -494,19 +601,6 @@ def main():
         [vertex_error(m, "gyroid", 256, meth) for meth in ("mt", "sn", "mc")]
     copy = copy_probe()
     cpu, cpu_all = (None, None) if a.no_cpu else cpu_baseline(m, a.n, a.cpu_planes)
-    # the emit kernel each autotuned family launches
-    knames = {"mc": {3: "k_emit_mc5", 2: "k_emit_mc5", 1: "k_emit_mc4", 0: "k_emit"},
-              "mt": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"},
-              "sn": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"}}[a.method]
-
-    def roofline(r):
-        b = r["field_bytes"] + r["out_bytes"]
-        ach = b / r["emit_s"]
-        return {"bound": "hbm", "achieved": round(ach / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4),
-                "traffic": traffic_of(a.method, a.n, r["dtype"]),
-                "kernel": knames.get(r["family_id"], "k_emit"), "algorithmic_bytes_per_launch": b,
-                "avg_launch_ms": round(r["emit_s"] * 1e3, 4)}
 
     def summary(r):
         b = r["field_bytes"] + r["out_bytes"]
@@ -514,9 +608,10 @@ def main():
                     ms_per_step=round(r["step_s"] * 1e3, 4),
                     count_ms=round(r["count_s"] * 1e3, 4), emit_ms=round(r["emit_s"] * 1e3, 4),
                     nverts=r["nverts"], nfaces=r["nfaces"], algorithmic_bytes=b,
-                    autotuned_emit_kernels=r["family"],
+                    autotuned_emit_kernels=r["family"], family_id=r["family_id"],
                     step_hbm_frac=round(b / r["step_s"] / HBM_PEAK, 4),
-                    roofline=roofline(r), weld=r["weld"])
+                    roofline=roofline(r, a.method, f"{a.method}_value_noise_{a.n}_{r['dtype']}"),
+                    weld=r["weld"])
 
     out = {
         "metric": "Gvoxels/s + % HBM roofline (1024^3 MC)",
@@ -535,7 +630,7 @@ def main():
                                f"Float64 {a.n}^3 value noise (32^3 lattice + 0.25 x 128^3 "
                                "octave, seed 1), X=Y=Z=-1:1",
                    "shape": [a.n] * 3, "seed": 1},
-        "roofline": roofline(main_r),
+        "roofline": roofline(main_r, a.method, f"{a.method}_value_noise_{a.n}_float64"),
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "hbm_copy_probe": copy,
