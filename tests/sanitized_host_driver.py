@@ -14,7 +14,7 @@ ROOT = Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(ROOT / "meshing.jl_amd"))
 import _capi as capi  # noqa: E402  (only the struct layouts and constants)
 
-L = ctypes.CDLL(str(ROOT / "meshing.jl_amd" / "libmeshing_hip_san.so"))
+L = ctypes.CDLL(sys.argv[1])  # the sanitizer build of libmeshing_hip.so
 vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
 pf, pp = ctypes.POINTER(capi.MshField), ctypes.POINTER(capi.MshParams)
 L.msh_output_kind.argtypes = [pf, pp, ctypes.POINTER(i32)]

@@ -34,22 +34,23 @@ def _run(cmd, preload, extra_env=None, timeout=600):
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
-def test_oracle_under_sanitizers():
-    subprocess.run(["make", "-C", str(ROOT / "oracle"), "sanitize"], check=True,
-                   capture_output=True)
+def test_oracle_under_sanitizers(tmp_path):
+    subprocess.run(["make", "-C", str(ROOT / "oracle"), "sanitize", f"SANDIR={tmp_path}"],
+                   check=True, capture_output=True)
     r = _run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:xdist", "-m", "not gpu",
               "tests/test_oracle.py", "tests/test_golden.py", "tests/test_tables.py",
               "tests/test_weld_oracle.py"],
              [_gcc_lib("libasan.so"), _gcc_lib("libstdc++.so")],
-             {"MSH_ORACLE_LIB": str(ROOT / "oracle" / "liboracle_san.so")})
+             {"MSH_ORACLE_LIB": str(tmp_path / "liboracle_san.so")})
     assert " passed" in r.stdout
 
 
 @pytest.mark.skipif(not CLANG_ASAN or shutil.which("/opt/rocm/bin/hipcc") is None,
                     reason="needs hipcc and its sanitizer runtime")
-def test_host_planning_under_sanitizers():
-    subprocess.run(["make", "-C", str(ROOT / "meshing.jl_amd"), "sanitize"], check=True,
-                   capture_output=True)
-    r = _run([sys.executable, str(ROOT / "tests" / "sanitized_host_driver.py")],
+def test_host_planning_under_sanitizers(tmp_path):
+    subprocess.run(["make", "-C", str(ROOT / "meshing.jl_amd"), "sanitize", f"SANDIR={tmp_path}"],
+                   check=True, capture_output=True)
+    r = _run([sys.executable, str(ROOT / "tests" / "sanitized_host_driver.py"),
+              str(tmp_path / "libmeshing_hip_san.so")],
              [str(CLANG_ASAN[-1]), _gcc_lib("libstdc++.so")])
     assert "no reports" in r.stdout
