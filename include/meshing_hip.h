@@ -151,6 +151,27 @@ int msh_weld(void* ctx, int32_t kind, int64_t nverts, const void* verts, int64_t
 int msh_generate_field(void* ctx, int32_t kind, int32_t dtype, int64_t nx, int64_t ny,
                        int64_t nz, uint64_t seed, double p0, double p1, void* out_device);
 
+/* Extraction that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It is
+ * msh_count followed by msh_extract in one call, on a device field into
+ * caller-owned device outputs with room for vcap vertices and fcap faces;
+ * vertices beyond vcap and faces beyond fcap are not written.  The call
+ * queues its work on the context's stream and returns without waiting; when
+ * that work has run, sizes[0] and sizes[1] (device memory) hold the exact
+ * vertex and face counts, so a caller whose capacity was short sees it there.
+ * The count/scan workspace is a static device array of
+ * MSH_NOALLOC_SCRATCH_BYTES per GPU, shared by every call on that GPU and
+ * holding the last call's contents: calls that overlap on one GPU (two
+ * contexts, or two streams) race on it and must be serialised by the caller.
+ * A volume whose workspace does not fit the array (about 20 bytes per z row
+ * of cells, so up to about 1.5 M rows, e.g. 1024 x 1024 x any nz) fails
+ * with MSH_ERR_ARGUMENT, as do host fields.  Results equal msh_extract's. */
+#define MSH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int msh_isosurface_noalloc(void* ctx, const msh_field* field, const msh_params* params,
+                           void* verts, int64_t vcap, int64_t* faces, int64_t fcap,
+                           int64_t* sizes);
+
 #ifdef __cplusplus
 }
 #endif

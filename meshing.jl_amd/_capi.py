@@ -20,7 +20,7 @@ MSH_HOST, MSH_DEVICE = range(2)
 EXPORTS = ("msh_create", "msh_destroy", "msh_set_stream", "msh_synchronize",
            "msh_last_error", "msh_output_kind", "msh_count", "msh_extract",
            "msh_batch_count", "msh_batch_extract", "msh_autotune", "msh_set_kernel_family",
-           "msh_generate_field", "msh_weld")
+           "msh_generate_field", "msh_weld", "msh_isosurface_noalloc")
 
 
 class MshField(ctypes.Structure):
@@ -95,6 +95,7 @@ def lib():
             L.msh_generate_field.argtypes = [vp, i32, i32, i64, i64, i64, ctypes.c_uint64,
                                              ctypes.c_double, ctypes.c_double, vp]
             L.msh_weld.argtypes = [vp, i32, i64, vp, i64, vp, vp, ctypes.POINTER(i64)]
+            L.msh_isosurface_noalloc.argtypes = [vp, pf, pp, vp, i64, vp, i64, vp]
             for name in EXPORTS:
                 f = getattr(L, name)
                 if name != "msh_last_error":

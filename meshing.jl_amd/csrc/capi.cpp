@@ -63,6 +63,7 @@ struct Ctx {
   hipEvent_t upload = nullptr;  // last descriptor upload from hbatch
   bool upload_pending = false;
   int64_t* pinned = nullptr;  // 4 x int64 for the totals
+  void* scratch = nullptr;    // this device's static scratch (msh_isosurface_noalloc)
   void* wws = nullptr;        // weld: hash table, scan workspace, indices
   size_t wws_bytes = 0;
   // emit kernel family per (method, element type), -1 = default order; set
@@ -345,7 +346,11 @@ int msh_create(int device, void** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->pinned, 4 * sizeof(int64_t), hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->upload, hipEventDisableTiming);
+  // once per device, with the device current: the address of its copy of
+  // the static scratch array
+  if (e == hipSuccess) e = noalloc_setup(device, &c->scratch);
   if (e != hipSuccess) {
+    if (c->upload) (void)hipEventDestroy(c->upload);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
@@ -500,6 +505,52 @@ int msh_extract(void* ctx, const msh_field* f, const msh_params* p, void* verts,
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "output download");
   }
+  return MSH_OK;
+}
+
+int msh_isosurface_noalloc(void* ctx, const msh_field* f, const msh_params* p, void* verts,
+                           int64_t vcap, int64_t* faces, int64_t fcap, int64_t* sizes) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  Plan pl;
+  int rc = make_plan(c, f, p, &pl);
+  if (rc) return rc;
+  pl.family = c->family[pl.method][pl.T];
+  if (f->memory != MSH_DEVICE)
+    return fail(c, MSH_ERR_ARGUMENT, "msh_isosurface_noalloc takes a device field");
+  if (vcap < 0 || fcap < 0) return fail(c, MSH_ERR_ARGUMENT, "negative output capacity");
+  if ((vcap > 0 && !verts) || (fcap > 0 && !faces))
+    return fail(c, MSH_ERR_ARGUMENT, "null output buffer");
+  if (!c->scratch) return fail(c, MSH_ERR_DEVICE, "no static scratch on this device");
+  const Geom g = make_geom(f);
+  // the workspace, then the clamped sizes the emit kernels read
+  const size_t need = workspace_layout(g.units, (int64_t)g.nx + g.ny + g.nz, nullptr, nullptr);
+  if (need + 256 > kScratchBytes)
+    return fail(c, MSH_ERR_ARGUMENT,
+                "volume needs " + std::to_string(need + 256) + " bytes of workspace, more than "
+                "the " + std::to_string(kScratchBytes) + "-byte static scratch");
+  if (g.units > 0 && !f->data) return fail(c, MSH_ERR_ARGUMENT, "null field data");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
+  Workspace w{};
+  workspace_layout(g.units, (int64_t)g.nx + g.ny + g.nz, c->scratch, &w);
+  int64_t* clamp = (int64_t*)((char*)c->scratch + need);
+  if (g.units == 0) {  // no cells: nothing to write, zero sizes
+    const hipError_t e = launch_noalloc_sizes(nullptr, vcap, fcap, clamp, sizes, c->stream);
+    return e == hipSuccess ? MSH_OK : hip_fail(c, e, "sizes");
+  }
+  VolArgs a = make_vol(pl, g, f->data, w);
+  hipError_t e = launch_coords(pl, g, w.coords, c->stream);
+  if (e == hipSuccess) e = launch_count(pl, a, c->stream);
+  if (e == hipSuccess) e = launch_scan(g, w, c->stream);
+  if (e == hipSuccess) e = launch_noalloc_sizes(w.totals, vcap, fcap, clamp, sizes, c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "count/scan");
+  a.sz = clamp;
+  a.verts = verts;
+  a.faces = faces;
+  e = launch_emit(pl, a, c->stream);
+  if (e == hipErrorNotSupported)
+    return fail(c, MSH_ERR_UNSUPPORTED, "this method is not available on the GPU path yet");
+  if (e != hipSuccess) return hip_fail(c, e, "emit");
   return MSH_OK;
 }
 

@@ -153,4 +153,13 @@ hipError_t launch_weld(const WeldArgs& a, hipStream_t st);
 hipError_t launch_generate(int kind, int dtype, int64_t nx, int64_t ny, int64_t nz,
                            uint64_t seed, double p0, double p1, void* out, hipStream_t st);
 
+// The static device scratch of msh_isosurface_noalloc (noalloc.hip).
+constexpr size_t kScratchBytes = MSH_NOALLOC_SCRATCH_BYTES;
+// This device's address of the scratch array; call with `device` current.
+hipError_t noalloc_setup(int device, void** ptr);
+// clamp = (0, min(nverts, vcap), 0, min(nfaces, fcap)) from the scan totals,
+// and sizes (when not null) = (nverts, nfaces); null totals count as zero.
+hipError_t launch_noalloc_sizes(const int64_t* totals, int64_t vcap, int64_t fcap, int64_t* clamp,
+                                int64_t* sizes, hipStream_t st);
+
 }  // namespace msh
