@@ -180,8 +180,10 @@ class BatchRunner:
         self.P = (capi.MshParams * n)(*[m._params(me, (-1, 1), (-1, 1), (-1, 1))
                                          for me in methods])
         self.ctx = capi.context(fields[0].device.index)
-        self.ctx.check(L.msh_set_stream(self.ctx.handle,
-                                        ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        # a stream of its own (default priority): a repeated batch then runs
+        # each phase from a captured hipGraph, which the null stream cannot hold
+        self.stream = torch.cuda.Stream(device=fields[0].device)
+        self.ctx.check(L.msh_set_stream(self.ctx.handle, ctypes.c_void_p(self.stream.cuda_stream)))
         self.nv = (ctypes.c_int64 * n)()
         self.nf = (ctypes.c_int64 * n)()
         self.ctx.check(L.msh_batch_count(self.ctx.handle, n, self.F, self.P, self.nv, self.nf))
@@ -200,6 +202,7 @@ class BatchRunner:
 
     def step(self):
         L, c = self.L, self.ctx
+        c.check(L.msh_set_stream(c.handle, ctypes.c_void_p(self.stream.cuda_stream)))
         c.check(L.msh_batch_count(c.handle, self.n, self.F, self.P, self.nv, self.nf))
         c.check(L.msh_batch_extract(c.handle, self.n, self.F, self.P, self.vp, self.fp,
                                     self.capi.MSH_DEVICE))
@@ -253,7 +256,11 @@ def measure_batch(m, nvol, steps, seed=7):
         torch.cuda.synchronize()
         t_batch.append(time.perf_counter() - t0)
     tp, tb = min(t_plain), min(t_batch)
+    g = (ctypes.c_int64(), ctypes.c_int64())
+    br.ctx.check(br.L.msh_batch_graph_launches(br.ctx.handle, ctypes.byref(g[0]),
+                                               ctypes.byref(g[1])))
     res = dict(volumes=nvol, samples=samples, seed=seed,
+               graph_launches={"count_phase": g[0].value, "emit_phase": g[1].value},
                faces=sum(br.nf[v] for v in range(nvol)),
                plain_ms=round(tp * 1e3, 3), batched_ms=round(tb * 1e3, 3),
                plain_req_s=round(nvol / tp, 1), batched_req_s=round(nvol / tb, 1),
@@ -399,6 +406,32 @@ def vertex_error(m, kind, n, method_name):
     del V32, F32, V64, F64
     torch.cuda.empty_cache()
     return res
+
+
+def drop_in_host(m, n=1024, dtype="float32"):
+    """The drop-in call as the reference is called: a host array in, host
+    arrays out (PCIe upload of the field, count, emit, download of the
+    mesh), on the same value-noise field; best of 2 after a warm-up call."""
+    import numpy as np
+    import torch
+    f = m.generate_field("value_noise", (n, n, n), dtype, seed=1)
+    host = np.asfortranarray(f.cpu().numpy())  # [x, y, z], x fastest
+    del f
+    torch.cuda.empty_cache()
+    meth = m.MarchingCubes(iso=0)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        V, F = m.isosurface(host, meth)
+        ts.append(time.perf_counter() - t0)
+        nv, nf, vb = len(V), len(F), V.nbytes + F.nbytes
+        del V, F
+    t = min(ts[1:])
+    del host
+    return dict(workload=f"MarchingCubes(iso=0), {dtype} {n}^3 value noise, numpy field in, "
+                         "numpy mesh out", seconds=round(t, 4), gvoxels_s=round(n ** 3 / t / 1e9, 3),
+                field_bytes=n ** 3 * (4 if dtype == "float32" else 8), mesh_bytes=vb,
+                nverts=nv, nfaces=nf)
 
 
 def copy_probe():
@@ -599,6 +632,7 @@ def main():
              if a.serve_requests else None)
     verr = [vertex_error(m, "value_noise", a.n, a.method)] + \
         [vertex_error(m, "gyroid", 256, meth) for meth in ("mt", "sn", "mc")]
+    host = drop_in_host(m, a.n) if a.n >= 256 else None
     copy = copy_probe()
     cpu, cpu_all = (None, None) if a.no_cpu else cpu_baseline(m, a.n, a.cpu_planes)
 
@@ -634,6 +668,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "hbm_copy_probe": copy,
+        "drop_in_host_arrays": host,
         "gyroid_256_f32": gyroid,
         "batched": batch,
         "serving": serve,

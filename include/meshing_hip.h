@@ -109,6 +109,16 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
                     int64_t* nverts, int64_t* nfaces);
 int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
                       void* const* verts, int64_t* const* faces, int32_t out_memory);
+/* A batch whose arguments repeat the previous call's (same fields, params,
+ * kernel choices and workspace buffers, device fields, device outputs, a
+ * stream other than the null stream) runs each phase from a hipGraph: the
+ * second such call captures the phase's copies and launches, later calls
+ * replay it (new output pointers travel through the descriptor copy, so
+ * they may change).  The count and emit kernels of the batch are persistent:
+ * as many blocks as fit on the device take the batch's tiles from a work
+ * queue.  msh_batch_graph_launches reports how many times each phase has run
+ * from a graph on this context.  No counterpart in the reference. */
+int msh_batch_graph_launches(void* ctx, int64_t* count_phase, int64_t* emit_phase);
 
 /* Kernel choice.  The emit pass has kernel families per method: 2 lanes
  * along x (the default), 1 lanes along z, 0 a generic kernel, and 3 the

@@ -20,7 +20,8 @@ MSH_HOST, MSH_DEVICE = range(2)
 EXPORTS = ("msh_create", "msh_destroy", "msh_set_stream", "msh_synchronize",
            "msh_last_error", "msh_output_kind", "msh_count", "msh_extract",
            "msh_batch_count", "msh_batch_extract", "msh_autotune", "msh_set_kernel_family",
-           "msh_generate_field", "msh_weld", "msh_isosurface_noalloc")
+           "msh_generate_field", "msh_weld", "msh_isosurface_noalloc",
+           "msh_batch_graph_launches")
 
 
 class MshField(ctypes.Structure):
@@ -96,6 +97,7 @@ def lib():
                                              ctypes.c_double, ctypes.c_double, vp]
             L.msh_weld.argtypes = [vp, i32, i64, vp, i64, vp, vp, ctypes.POINTER(i64)]
             L.msh_isosurface_noalloc.argtypes = [vp, pf, pp, vp, i64, vp, i64, vp]
+            L.msh_batch_graph_launches.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
             for name in EXPORTS:
                 f = getattr(L, name)
                 if name != "msh_last_error":

@@ -13,6 +13,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <string>
 #include <vector>
@@ -32,6 +33,17 @@ struct Launch {
   int64_t nblocks = 0;
 };
 
+// A captured phase of the batched form.  A batch whose arguments and
+// device buffers repeat those of the previous call runs its device work from
+// a hipGraph: the second such call captures it, later ones replay it.  The
+// key is the bytes that decide every launch and copy of the phase.
+struct PhaseGraph {
+  std::vector<char> last;   // key of the previous call of this phase
+  std::vector<char> key;    // key the graph was captured for
+  hipGraphExec_t exec = nullptr;
+  int64_t replays = 0;
+};
+
 // State of the last successful msh_batch_count.
 struct BatchState {
   bool have = false;
@@ -42,6 +54,8 @@ struct BatchState {
   std::vector<Launch> counts, emits;
   std::vector<int64_t> sizes;      // 4 per volume: first vertex, nverts, first face, nfaces
   int64_t units = 0, ncoords = 0, cblocks = 0, eblocks = 0;
+  size_t ctr_off = 0;              // work-queue counters in the batch workspace: 8 per
+                                   // count launch, then 8 per emit launch
 };
 
 struct Ctx {
@@ -79,7 +93,54 @@ struct Ctx {
   VolArgs vol{};
   int64_t nverts = 0, nfaces = 0;
   BatchState batch;
+  PhaseGraph gcount, gemit;  // batched count and emit phases
 };
+
+void graph_drop(PhaseGraph* g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  g->exec = nullptr;
+  g->key.clear();
+}
+
+template <class T>
+void key_add(std::vector<char>* k, const T* p, size_t n) {
+  const char* b = (const char*)p;
+  k->insert(k->end(), b, b + sizeof(T) * n);
+}
+
+// Run `enqueue` (device work on c->stream) eagerly, or through the phase's
+// graph: replayed when `key` matches the captured one, captured when it
+// matches the previous call's (a batch that repeats).  `graphable` is false
+// for work a graph must not hold (pageable host copies, the null stream).
+template <class F>
+hipError_t run_phase(Ctx* c, PhaseGraph* g, std::vector<char> key, bool graphable, F enqueue) {
+  hipError_t e = hipSuccess;
+  const bool repeat = graphable && key == g->last;
+  g->last = key;
+  if (!repeat) return enqueue();
+  if (!g->exec || g->key != key) {
+    graph_drop(g);
+    hipGraph_t graph = nullptr;
+    e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return enqueue();  // not capturable here: run eagerly
+    const hipError_t eq = enqueue();
+    e = hipStreamEndCapture(c->stream, &graph);
+    if (eq != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return eq;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {  // nothing ran: run the phase eagerly
+      g->exec = nullptr;
+      (void)hipGetLastError();
+      return enqueue();
+    }
+    g->key = key;
+  }
+  ++g->replays;
+  return hipGraphLaunch(g->exec, c->stream);
+}
 
 int fail(Ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -373,6 +434,8 @@ int msh_destroy(void* ctx) {
   if (c->bws) (void)hipFree(c->bws);
   if (c->wws) (void)hipFree(c->wws);
   if (c->hbatch) (void)hipHostFree(c->hbatch);
+  graph_drop(&c->gcount);
+  graph_drop(&c->gemit);
   if (c->upload) (void)hipEventDestroy(c->upload);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own) (void)hipStreamDestroy(c->own);
@@ -624,14 +687,21 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
   }
   b.counts.clear();
   b.emits.clear();
+  // Within a launch, the volumes' tiles are queued longest first (a tile
+  // walks the volume's z extent), so the launch does not end waiting on a
+  // long tile taken last.
+  std::vector<int> ord((size_t)n);
+  for (int v = 0; v < n; ++v) ord[v] = v;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](int x, int y) { return geoms[x].nz > geoms[y].nz; });
   std::vector<char> done((size_t)n, 0);
   int64_t cblocks = 0;
-  for (int v = 0; v < n; ++v) {
+  for (int v : ord) {
     if (done[v]) continue;
     Launch L;
     L.plan = b.plans[v];
     L.gbase = cblocks;
-    for (int u = v; u < n; ++u) {
+    for (int u : ord) {
       if (done[u] || b.plans[u].T != L.plan.T || b.plans[u].method != L.plan.method) continue;
       done[u] = 1;
       b.vols[u].cblk0 = cblocks;
@@ -642,13 +712,13 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
   }
   done.assign((size_t)n, 0);
   int64_t eblocks = 0;
-  for (int v = 0; v < n; ++v) {
+  for (int v : ord) {
     if (done[v]) continue;
     Launch L;
     L.plan = b.plans[v];
     L.variant = emit_var[v];
     L.gbase = eblocks;
-    for (int u = v; u < n; ++u) {
+    for (int u : ord) {
       if (done[u] || emit_var[u] != L.variant || !same_plan_types(b.plans[u], L.plan)) continue;
       done[u] = 1;
       b.vols[u].eblk0 = eblocks;
@@ -665,12 +735,17 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
   const size_t b_cmap = align256(4 * (size_t)(cblocks + 1));
   const size_t b_emap = align256(4 * (size_t)(eblocks + 1));
   const size_t b_sizes = align256(32 * (size_t)(n > 0 ? n : 1));
-  rc = grow(c, &c->bws, &c->bws_bytes, b_vols + b_cmap + b_emap + b_sizes, "batch workspace");
+  const size_t nctr = 8 * (b.counts.size() + b.emits.size());
+  const size_t b_ctr = align256(4 * (nctr + 1));
+  rc = grow(c, &c->bws, &c->bws_bytes, b_vols + b_cmap + b_emap + b_sizes + b_ctr,
+            "batch workspace");
   if (rc) return rc;
   VolArgs* dvols = (VolArgs*)c->bws;
   int32_t* cmap = (int32_t*)((char*)c->bws + b_vols);
   int32_t* emap = (int32_t*)((char*)cmap + b_cmap);
   int64_t* dsizes = (int64_t*)((char*)emap + b_emap);
+  b.ctr_off = b_vols + b_cmap + b_emap + b_sizes;
+  uint32_t* ctrs = (uint32_t*)((char*)c->bws + b.ctr_off);
   for (int v = 0; v < n; ++v) b.vols[v].sz = dsizes + 4 * v;
   // pinned host side: descriptors | sizes
   if (c->hbatch_bytes < b_vols + b_sizes) {
@@ -684,27 +759,44 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
   int64_t* hsizes = (int64_t*)((char*)c->hbatch + b_vols);
   if (n > 0) memcpy(c->hbatch, b.vols.data(), sizeof(VolArgs) * (size_t)n);
 
-  hipError_t e = hipSuccess;
-  for (int v = 0; v < n && e == hipSuccess; ++v)
-    if (fields[v].memory == MSH_HOST && geoms[v].units > 0)
-      e = hipMemcpyAsync((char*)c->fstage + hoff[v], fields[v].data,
-                         (size_t)fields[v].nx * fields[v].ny * fields[v].nz *
-                             (fields[v].dtype == K_F32 ? 4 : 8),
-                         hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess && n > 0)
-    e = hipMemcpyAsync(dvols, c->hbatch, sizeof(VolArgs) * (size_t)n, hipMemcpyHostToDevice,
-                       c->stream);
-  if (e == hipSuccess) e = launch_fill_maps(dvols, n, cmap, emap, c->stream);
-  if (e == hipSuccess) e = launch_coords_batch(dvols, n, c->stream);
-  for (size_t k = 0; k < b.counts.size() && e == hipSuccess; ++k)
-    e = launch_count_batch(b.counts[k].plan, dvols, cmap, b.counts[k].gbase,
-                           b.counts[k].nblocks, c->stream);
-  Geom gall{};
-  gall.units = units;
-  if (e == hipSuccess && units > 0) e = launch_scan(gall, w, c->stream);
-  if (e == hipSuccess && units > 0) e = launch_sizes_batch(dvols, n, w, units, dsizes, c->stream);
-  if (e == hipSuccess && units > 0)
-    e = hipMemcpyAsync(hsizes, dsizes, 32 * (size_t)n, hipMemcpyDeviceToHost, c->stream);
+  bool host_fields = false;
+  for (int v = 0; v < n; ++v) host_fields |= fields[v].memory == MSH_HOST && geoms[v].units > 0;
+  auto enqueue = [&]() -> hipError_t {
+    hipError_t e = hipSuccess;
+    for (int v = 0; v < n && e == hipSuccess; ++v)
+      if (fields[v].memory == MSH_HOST && geoms[v].units > 0)
+        e = hipMemcpyAsync((char*)c->fstage + hoff[v], fields[v].data,
+                           (size_t)fields[v].nx * fields[v].ny * fields[v].nz *
+                               (fields[v].dtype == K_F32 ? 4 : 8),
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && n > 0)
+      e = hipMemcpyAsync(dvols, c->hbatch, sizeof(VolArgs) * (size_t)n, hipMemcpyHostToDevice,
+                         c->stream);
+    if (e == hipSuccess) e = launch_fill_maps(dvols, n, cmap, emap, c->stream);
+    if (e == hipSuccess) e = launch_coords_batch(dvols, n, c->stream);
+    if (e == hipSuccess) e = launch_zero_u32(ctrs, (int64_t)(8 * b.counts.size()), c->stream);
+    for (size_t k = 0; k < b.counts.size() && e == hipSuccess; ++k)
+      e = launch_count_batch(b.counts[k].plan, dvols, cmap, b.counts[k].gbase,
+                             b.counts[k].nblocks, ctrs + 8 * k, c->stream);
+    Geom gall{};
+    gall.units = units;
+    if (e == hipSuccess && units > 0) e = launch_scan(gall, w, c->stream);
+    if (e == hipSuccess && units > 0)
+      e = launch_sizes_batch(dvols, n, w, units, dsizes, c->stream);
+    if (e == hipSuccess && units > 0)
+      e = hipMemcpyAsync(hsizes, dsizes, 32 * (size_t)n, hipMemcpyDeviceToHost, c->stream);
+    return e;
+  };
+  // everything the launches and copies above are derived from: the
+  // arguments, the kernel family choices and the buffers
+  std::vector<char> key;
+  key_add(&key, fields, (size_t)n);
+  key_add(&key, params, (size_t)n);
+  key_add(&key, &c->family[0][0], 9);
+  const void* bufs[5] = {c->ws, c->bws, c->hbatch, c->fstage, (const void*)c->stream};
+  key_add(&key, bufs, 5);
+  hipError_t e = run_phase(c, &c->gcount, std::move(key), !host_fields && c->stream != nullptr,
+                           enqueue);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "batched count/scan");
 
@@ -772,13 +864,29 @@ int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_p
   const size_t b_cmap = align256(4 * (size_t)(b.cblocks + 1));
   int32_t* emap = (int32_t*)((char*)c->bws + b_vols + b_cmap);
   memcpy(c->hbatch, b.vols.data(), sizeof(VolArgs) * (size_t)n);
-  hipError_t e = hipMemcpyAsync(dvols, c->hbatch, sizeof(VolArgs) * (size_t)n,
-                                hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = hipEventRecord(c->upload, c->stream);
+  uint32_t* ectr = (uint32_t*)((char*)c->bws + b.ctr_off) + 8 * b.counts.size();
+  // the output pointers travel in the descriptors (copied from hbatch by
+  // the first operation), so a captured emit phase serves new outputs
+  auto enqueue = [&]() -> hipError_t {
+    hipError_t e = hipMemcpyAsync(dvols, c->hbatch, sizeof(VolArgs) * (size_t)n,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->upload, c->stream);
+    if (e == hipSuccess) e = launch_zero_u32(ectr, (int64_t)(8 * b.emits.size()), c->stream);
+    for (size_t k = 0; k < b.emits.size() && e == hipSuccess; ++k)
+      e = launch_emit_batch(b.emits[k].plan, b.emits[k].variant, dvols, emap, b.emits[k].gbase,
+                            b.emits[k].nblocks, ectr + 8 * k, c->stream);
+    return e;
+  };
+  std::vector<char> key;
+  key_add(&key, b.fields.data(), (size_t)n);
+  key_add(&key, b.params.data(), (size_t)n);
+  key_add(&key, &c->family[0][0], 9);
+  const void* bufs[5] = {c->ws, c->bws, c->hbatch, (const void*)c->stream,
+                         (const void*)c->upload};
+  key_add(&key, bufs, 5);
+  hipError_t e = run_phase(c, &c->gemit, std::move(key),
+                           out_memory == MSH_DEVICE && c->stream != nullptr, enqueue);
   if (e == hipSuccess) c->upload_pending = true;
-  for (size_t k = 0; k < b.emits.size() && e == hipSuccess; ++k)
-    e = launch_emit_batch(b.emits[k].plan, b.emits[k].variant, dvols, emap, b.emits[k].gbase,
-                          b.emits[k].nblocks, c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "batched emit");
   if (out_memory == MSH_HOST) {
     for (int v = 0; v < n && e == hipSuccess; ++v) {
@@ -789,6 +897,14 @@ int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_p
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "output download");
   }
+  return MSH_OK;
+}
+
+int msh_batch_graph_launches(void* ctx, int64_t* count_phase, int64_t* emit_phase) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  if (count_phase) *count_phase = c->gcount.replays;
+  if (emit_phase) *emit_phase = c->gemit.replays;
   return MSH_OK;
 }
 

@@ -72,6 +72,66 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 
 
 
+// Tile sources of the kernels' tile loops.  A single-volume launch runs one
+// tile per block (OneTile).  A batched launch is persistent: as many blocks
+// as fit on the device at once, each taking tiles from a work queue until
+// the batch's tiles are gone (TileQueue), so the per-block set-up (lookup
+// tables in LDS) is paid once per block rather than once per tile, and
+// volumes of different sizes balance across the CUs.  The launch's tiles are
+// cut into 8 contiguous ranges, one per XCD (block b starts on XCD b % 8);
+// a block takes tiles from its own XCD's range first, so tiles that are
+// neighbours in the batch (and share cache lines at their edges) meet in
+// that XCD's L2, and then helps the other ranges in turn.  Each range has
+// its own counter (ctr[0..7], zero at launch).
+struct OneTile {
+  static constexpr bool kOne = true;  // the kernels call the tile directly
+  const VolArgs* p;
+  int64_t t;
+  bool done;
+  __device__ bool operator()(VolArgs& a, int64_t& tile) {
+    if (done) return false;
+    done = true;
+    a = *p;
+    tile = t;
+    return true;
+  }
+};
+
+template <bool EMIT>
+struct TileQueue {
+  static constexpr bool kOne = false;
+  const VolArgs* __restrict__ vols;
+  const int32_t* __restrict__ map;  // block of the batch -> volume
+  int64_t gbase, n;                 // the launch's first block of the batch, its tiles
+  uint32_t* __restrict__ ctr;       // 8 counters
+  int k;                            // ranges exhausted so far
+  __device__ bool operator()(VolArgs& a, int64_t& tile) {
+    __shared__ int64_t slot;
+    const int q0 = (int)(blockIdx.x & 7);
+    while (k < 8) {
+      const int q = (q0 + k) & 7;
+      const int64_t per = n >> 3, rem = n & 7;
+      const int64_t lo = q * per + (q < rem ? q : rem), cnt = per + (q < rem ? 1 : 0);
+      lds_barrier();  // every wave has read the previous slot
+      if (threadIdx.x == 0) slot = (int64_t)atomicAdd(&ctr[q], 1u);
+      lds_barrier();
+      // uniform across the block: keep it (and the descriptor loads) scalar
+      const uint64_t raw = (uint64_t)slot;
+      const int64_t i = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
+                                      (int)(raw >> 32)) << 32) |
+                                  (uint32_t)__builtin_amdgcn_readfirstlane((int)raw));
+      if (i < cnt) {
+        const int64_t b = gbase + lo + i;
+        a = vols[map[b]];
+        tile = b - (EMIT ? a.eblk0 : a.cblk0);
+        return true;
+      }
+      ++k;
+    }
+    return false;
+  }
+};
+
 // Inclusive scan over the 64 lanes with DPP: Kogge-Stone within each row of
 // 16 lanes, then row broadcasts of lanes 15 and 31.
 __device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
@@ -187,7 +247,7 @@ __device__ __forceinline__ void dma_to_lds(__amdgpu_buffer_rsrc_t r, uint32_t* l
 void count4_tiling(const Plan& p, VolArgs* a);
 hipError_t launch_count4(const Plan& p, const VolArgs& a, hipStream_t st);
 hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
-                               int64_t gbase, int64_t nblocks, hipStream_t st);
+                               int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st);
 // -1: volume too large for k_emit_mc4; else bit 0: dword DMA (not 16-byte),
 // bits 1-2: axis types (1 all Float64, 2 all Float32, 0 mixed), bits 3-4:
 // tile shape.  Fills the emit tiling.
@@ -199,7 +259,7 @@ int mc5_variant(const Plan& p, VolArgs* a);
 hipError_t launch_emit_mc5(const Plan& p, int variant, const VolArgs& a, hipStream_t st);
 hipError_t launch_emit_mc5_batch(const Plan& p, int variant, const VolArgs* vols,
                                  const int32_t* emap, int64_t gbase, int64_t nblocks,
-                                 hipStream_t st);
+                                 uint32_t* ctr, hipStream_t st);
 
 // halo5.hip: the Marching Tetrahedra and Naive Surface Nets emit kernel
 // with lanes along x.  -1: volume too large; else bit 0: dword DMA, bits
@@ -208,7 +268,11 @@ int h5_variant(const Plan& p, VolArgs* a);
 hipError_t launch_emit_h5(const Plan& p, int variant, const VolArgs& a, hipStream_t st);
 hipError_t launch_emit_h5_batch(const Plan& p, int variant, const VolArgs* vols,
                                 const int32_t* emap, int64_t gbase, int64_t nblocks,
-                                hipStream_t st);
+                                uint32_t* ctr, hipStream_t st);
+
+// Blocks of `kernel` (256 threads, static LDS only) that fit on the current
+// device at once, capped at `ntiles`: the grid of a persistent launch.
+unsigned persistent_grid(const void* kernel, int64_t ntiles);
 
 // tet.hip: the Marching Tetrahedra and Naive Surface Nets emit kernel.  -1:
 // volume too large; else bit 0: dword DMA, bits 1-2: axis types.  Fills the

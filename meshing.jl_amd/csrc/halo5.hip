@@ -26,8 +26,9 @@ namespace {
 constexpr int kH5TX = 56;  // cells a block emits along x: lanes 4..59
 constexpr int kH5CY = 4;   // cell rows a block emits along y: one per wave
 
-template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM>
-__device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
+template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM,
+          class Src>
+__device__ __forceinline__ void emit_h5_run(Src src) {
   static_assert(METHOD == MT || METHOD == SN, "tetrahedra or surface nets");
   static_assert(ZC <= 16, "z fits 4 bits of a queue entry");
   constexpr int SX = 64, SY = kH5CY + 2, NP = ZC + 2;  // planes zc-1 .. zc+ZC, rows jb-1 .. jb+4
@@ -59,27 +60,8 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
     uint64_t vbits[kWaves][CAPV / 64], fbits[kWaves][CAPF / 64];
   };
   __shared__ Smem sm;
-  const T* __restrict__ f = (const T*)a.f;
-  Out* __restrict__ verts = (Out*)a.verts;
-  int64_t* __restrict__ faces = a.faces;
-  const Geom g = a.g;
   const int lane = lane_id(), wv = wave_id();
-  const int ib = (int)(t % a.nti) * kH5TX;
-  const int64_t rest = t / a.nti;
-  const int jb = (int)(rest % a.ntj) * kH5CY;
-  const int s = (int)(rest / a.ntj);
-  const int zs = s * g.zseg, ze = min(zs + g.zseg, g.nz - 1);
-  const T thr = (T)a.thr;
-  const M iso = (M)a.iso;
-  const A eps = (A)a.eps, ome = (A)a.ome;
-  const int cxm = CXM >= 0 ? CXM : a.cxmask;
-  const bool cx32 = cxm & 1, cy32 = (cxm >> 1) & 1, cz32 = (cxm >> 2) & 1;
-  const double* zp = a.coords + g.nx + g.ny;
-  const int64_t vbase = a.sz[0], totv = a.sz[1], fbase = a.sz[2], totf = a.sz[3];
-  const int64_t pstride = (int64_t)g.nx * g.ny;
-  const int64_t nelem = pstride * g.nz;
-  const int x0 = ib - 4, y0 = jb - 1;  // brick origin: sample column 0, row 0
-
+  // lookup tables: the same for every tile
   if constexpr (METHOD == MT) {
     for (int c = threadIdx.x; c < 256; c += kBlock) {
       const uint32_t m = nat_to_mt((uint32_t)c);
@@ -128,411 +110,443 @@ __device__ __forceinline__ void emit_h5_tile(const VolArgs& a, int64_t t) {
       sm.xmask[c] = (uint16_t)m;
     }
   }
-  for (int q = threadIdx.x; q < SX; q += kBlock) sm.cx[q] = a.coords[min(max(x0 + q, 0), g.nx - 1)];
-  if (threadIdx.x < SY)
-    sm.cy[threadIdx.x] = a.coords[g.nx + min(max(y0 + (int)threadIdx.x, 0), g.ny - 1)];
 
-  // this lane's cell row (row slot sl), and for wave 0 the row at -y
-  const int sl = wv + 1;
-  const int gi = x0 + lane, gj = y0 + sl;
-  const bool colok = lane < SX - 1 && gi >= 0 && gi < g.nx - 1;
-  const bool exists = colok && gj < g.ny - 1;
-  const bool emits = exists && lane >= 4 && lane < 4 + kH5TX;
-  const bool hexists = wv == 0 && colok && y0 >= 0;
-  int64_t rbv = 0, rbf = 0, hrbv = 0;
-  if (exists) {
-    const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + gj) * g.nzseg + s;
-    rbv = a.voff[u] - vbase;
-    rbf = a.foff[u] - fbase;
-  }
-  if (hexists) hrbv = a.voff[a.ubase + ((int64_t)gi * (g.ny - 1) + y0) * g.nzseg + s] - vbase;
+  auto tile = [&](const VolArgs& a, const int64_t t) {
+    const T* __restrict__ f = (const T*)a.f;
+    Out* __restrict__ verts = (Out*)a.verts;
+    int64_t* __restrict__ faces = a.faces;
+    const Geom g = a.g;
+    const int ib = (int)(t % a.nti) * kH5TX;
+    const int64_t rest = t / a.nti;
+    const int jb = (int)(rest % a.ntj) * kH5CY;
+    const int s = (int)(rest / a.ntj);
+    const int zs = s * g.zseg, ze = min(zs + g.zseg, g.nz - 1);
+    const T thr = (T)a.thr;
+    const M iso = (M)a.iso;
+    const A eps = (A)a.eps, ome = (A)a.ome;
+    const int cxm = CXM >= 0 ? CXM : a.cxmask;
+    const bool cx32 = cxm & 1, cy32 = (cxm >> 1) & 1, cz32 = (cxm >> 2) & 1;
+    const double* zp = a.coords + g.nx + g.ny;
+    const int64_t vbase = a.sz[0], totv = a.sz[1], fbase = a.sz[2], totf = a.sz[3];
+    const int64_t pstride = (int64_t)g.nx * g.ny;
+    const int64_t nelem = pstride * g.nz;
+    const int x0 = ib - 4, y0 = jb - 1;  // brick origin: sample column 0, row 0
+    for (int q = threadIdx.x; q < SX; q += kBlock) sm.cx[q] = a.coords[min(max(x0 + q, 0), g.nx - 1)];
+    if (threadIdx.x < SY)
+      sm.cy[threadIdx.x] = a.coords[g.nx + min(max(y0 + (int)threadIdx.x, 0), g.ny - 1)];
 
-  // DMA: planes zc-1 .. zc+ZC, rows y0 .. y0+4, 64 samples from x0.  Near
-  // the volume's low faces the origin is outside the field: the buffer starts
-  // at the first sample that exists and lanes before it read 0.
-  constexpr int NROWS = NP * SY;
-  constexpr int RPI = VEC ? 1024 / (4 * RD) : 1;
-  constexpr int NI = VEC ? (NROWS + RPI - 1) / RPI : NROWS * DW;
-  constexpr int NIW = (NI + kWaves - 1) / kWaves;
-  uint32_t doff[NIW];
-  bool dok[NIW];
-#pragma unroll
-  for (int m = 0; m < NIW; ++m) {
-    const int ins = wv + m * kWaves;
-    int row, byte_in_row;
-    if (VEC) {
-      row = ins * RPI + lane / (kWave / RPI);
-      byte_in_row = (lane % (kWave / RPI)) * 16;
-    } else {
-      row = ins / DW;
-      byte_in_row = (ins % DW) * 256 + lane * 4;
+    // this lane's cell row (row slot sl), and for wave 0 the row at -y
+    const int sl = wv + 1;
+    const int gi = x0 + lane, gj = y0 + sl;
+    const bool colok = lane < SX - 1 && gi >= 0 && gi < g.nx - 1;
+    const bool exists = colok && gj < g.ny - 1;
+    const bool emits = exists && lane >= 4 && lane < 4 + kH5TX;
+    const bool hexists = wv == 0 && colok && y0 >= 0;
+    int64_t rbv = 0, rbf = 0, hrbv = 0;
+    if (exists) {
+      const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + gj) * g.nzseg + s;
+      rbv = a.voff[u] - vbase;
+      rbf = a.foff[u] - fbase;
     }
-    dok[m] = ins < NI && row < NROWS;
-    const int pl = row / SY, y = row - pl * SY;
-    doff[m] = (uint32_t)(((int64_t)pl * pstride + (int64_t)y * g.nx) * (int64_t)sizeof(T)) +
-              (uint32_t)byte_in_row;
-  }
-  auto fetch = [&](int zc, int buf) {
-    const int64_t start = (int64_t)(zc - 1) * pstride + (int64_t)y0 * g.nx + x0;
-    const int64_t first = start < 0 ? 0 : start;
-    const uint32_t skip = (uint32_t)((first - start) * (int64_t)sizeof(T));
-    const int64_t left = (nelem - first) * (int64_t)sizeof(T);
-    const __amdgpu_buffer_rsrc_t rs =
-        buffer_rsrc(f + first, (uint32_t)(left < 0x7FFFFFFF ? left : 0x7FFFFFFF));
-#pragma unroll
+    if (hexists) hrbv = a.voff[a.ubase + ((int64_t)gi * (g.ny - 1) + y0) * g.nzseg + s] - vbase;
+
+    // DMA: planes zc-1 .. zc+ZC, rows y0 .. y0+4, 64 samples from x0.  Near
+    // the volume's low faces the origin is outside the field: the buffer starts
+    // at the first sample that exists and lanes before it read 0.
+    constexpr int NROWS = NP * SY;
+    constexpr int RPI = VEC ? 1024 / (4 * RD) : 1;
+    constexpr int NI = VEC ? (NROWS + RPI - 1) / RPI : NROWS * DW;
+    constexpr int NIW = (NI + kWaves - 1) / kWaves;
+    uint32_t doff[NIW];
+    bool dok[NIW];
+  #pragma unroll
     for (int m = 0; m < NIW; ++m) {
       const int ins = wv + m * kWaves;
-      // 0x80000000 is past every buffer's range: those lanes read zeros
-      const uint32_t off = doff[m] >= skip ? doff[m] - skip : 0x80000000u;
-      if (dok[m]) dma_to_lds<VEC ? 16 : 4>(rs, sm.smp[buf] + ins * (VEC ? 256 : kWave), off);
-    }
-    constexpr int NZ = (2 * NP + kWave - 1) / kWave;
-    uint32_t* zd = (uint32_t*)sm.cz[buf];
-    for (int ins = wv; ins < NZ; ins += kWaves) {
-      const int p = ins * kWave + lane;
-      if (p < 2 * NP) {
-        const double* src = zp + min(max(zc - 1 + (p >> 1), 0), g.nz - 1);
-        __builtin_amdgcn_global_load_lds((const void*)((const char*)src + 4 * (p & 1)),
-                                         (void*)(zd + ins * kWave), 4, 0, 0);
+      int row, byte_in_row;
+      if (VEC) {
+        row = ins * RPI + lane / (kWave / RPI);
+        byte_in_row = (lane % (kWave / RPI)) * 16;
+      } else {
+        row = ins / DW;
+        byte_in_row = (ins % DW) * 256 + lane * 4;
       }
+      dok[m] = ins < NI && row < NROWS;
+      const int pl = row / SY, y = row - pl * SY;
+      doff[m] = (uint32_t)(((int64_t)pl * pstride + (int64_t)y * g.nx) * (int64_t)sizeof(T)) +
+                (uint32_t)byte_in_row;
+    }
+    auto fetch = [&](int zc, int buf) {
+      const int64_t start = (int64_t)(zc - 1) * pstride + (int64_t)y0 * g.nx + x0;
+      const int64_t first = start < 0 ? 0 : start;
+      const uint32_t skip = (uint32_t)((first - start) * (int64_t)sizeof(T));
+      const int64_t left = (nelem - first) * (int64_t)sizeof(T);
+      const __amdgpu_buffer_rsrc_t rs =
+          buffer_rsrc(f + first, (uint32_t)(left < 0x7FFFFFFF ? left : 0x7FFFFFFF));
+  #pragma unroll
+      for (int m = 0; m < NIW; ++m) {
+        const int ins = wv + m * kWaves;
+        // 0x80000000 is past every buffer's range: those lanes read zeros
+        const uint32_t off = doff[m] >= skip ? doff[m] - skip : 0x80000000u;
+        if (dok[m]) dma_to_lds<VEC ? 16 : 4>(rs, sm.smp[buf] + ins * (VEC ? 256 : kWave), off);
+      }
+      constexpr int NZ = (2 * NP + kWave - 1) / kWave;
+      uint32_t* zd = (uint32_t*)sm.cz[buf];
+      for (int ins = wv; ins < NZ; ins += kWaves) {
+        const int p = ins * kWave + lane;
+        if (p < 2 * NP) {
+          const double* src = zp + min(max(zc - 1 + (p >> 1), 0), g.nz - 1);
+          __builtin_amdgcn_global_load_lds((const void*)((const char*)src + 4 * (p & 1)),
+                                           (void*)(zd + ins * kWave), 4, 0, 0);
+        }
+      }
+    };
+
+    uint32_t* vlist = sm.vlist[wv];
+    uint32_t* flist = sm.flist[wv];
+    uint64_t* vbits = sm.vbits[wv];
+    uint64_t* fbits = sm.fbits[wv];
+    for (int q = lane; q < CAPV / 64; q += kWave) vbits[q] = 0;
+    for (int q = lane; q < CAPF / 64; q += kWave) fbits[q] = 0;
+
+    if (zs < ze) fetch(zs, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    lds_barrier();
+    int buf = 0;
+    for (int zc = zs; zc < ze; zc += ZC, buf ^= 1) {
+      if (zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
+      const uint32_t* smp = sm.smp[buf];
+      const double* czs = sm.cz[buf];
+
+      // classify this lane's row: natural-order face nibble per plane
+      uint32_t nib[NP];
+  #pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const T v0 = *(const T*)(smp + (p * SY + sl) * RD + lane * DW);
+        const T v1 = *(const T*)(smp + (p * SY + sl + 1) * RD + lane * DW);
+        const uint32_t w = (uint32_t)(v0 < thr) | ((uint32_t)(v1 < thr) << 2);
+        nib[p] = w | (next_lane(w, 0u) << 1);
+      }
+      const int zn = min(ZC, ze - zc);
+      uint32_t cs[ZC], nvz[ZC], nfz[ZC];
+      uint32_t runv = 0, tot = 0;  // in-row vertex prefix; emitted vertices | faces << 16
+  #pragma unroll
+      for (int z = 0; z < ZC; ++z) {
+        const int gk = zc + z;
+        cs[z] = (exists && z < zn) ? (nib[z + 1] | (nib[z + 2] << 4)) : 0u;
+        const uint32_t cls = (uint32_t)((gi == 0) | ((gj == 0) << 1) | ((gk == 0) << 2));
+        if constexpr (METHOD == MT) {
+          nvz[z] = sm.nown[cls * 256 + cs[z]];
+          nfz[z] = emits ? (uint32_t)sm.nf[cs[z]] : 0u;
+          sm.cc[sl][z][lane] = (uint8_t)cs[z];
+        } else {
+          const uint32_t csm = nat_to_mc(cs[z]), c0 = csm & 1;
+          nvz[z] = (cs[z] != 0 && cs[z] != 255) ? 1u : 0u;
+          const uint32_t nq = (uint32_t)((c0 != ((csm >> 1) & 1)) && gj >= 1 && gk >= 1) +
+                              (uint32_t)((c0 != ((csm >> 3) & 1)) && gk >= 1 && gi >= 1) +
+                              (uint32_t)((c0 != ((csm >> 4) & 1)) && gi >= 1 && gj >= 1);
+          nfz[z] = (emits && nvz[z]) ? 2u * nq : 0u;
+        }
+        sm.cp[sl][z][lane] = (uint16_t)runv;
+        runv += nvz[z];
+        if (emits) tot += nvz[z] | (nfz[z] << 16);
+      }
+      if constexpr (METHOD == MT) sm.pc[sl][lane] = (uint8_t)((exists && zc > 0) ? (nib[0] | (nib[1] << 4)) : 0u);
+      sm.rb[sl][lane] = rbv;
+      if (wv == 0) {
+        // the row at -y (slot 0): cases and vertex prefixes for the owner
+        // lookups of row jb's faces; it emits nothing here
+        uint32_t hn[NP];
+  #pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          const T v0 = *(const T*)(smp + (p * SY) * RD + lane * DW);
+          const T v1 = *(const T*)(smp + (p * SY + 1) * RD + lane * DW);
+          const uint32_t w = (uint32_t)(v0 < thr) | ((uint32_t)(v1 < thr) << 2);
+          hn[p] = w | (next_lane(w, 0u) << 1);
+        }
+        uint32_t hrun = 0;
+  #pragma unroll
+        for (int z = 0; z < ZC; ++z) {
+          const uint32_t c = (hexists && z < zn) ? (hn[z + 1] | (hn[z + 2] << 4)) : 0u;
+          uint32_t nv;
+          if constexpr (METHOD == MT) {
+            const uint32_t cls = (uint32_t)((gi == 0) | ((y0 == 0) << 1) | ((zc + z == 0) << 2));
+            nv = sm.nown[cls * 256 + c];
+            sm.cc[0][z][lane] = (uint8_t)c;
+          } else {
+            nv = (c != 0 && c != 255) ? 1u : 0u;
+          }
+          sm.cp[0][z][lane] = (uint16_t)hrun;
+          hrun += nv;
+        }
+        if constexpr (METHOD == MT) sm.pc[0][lane] = (uint8_t)((hexists && zc > 0) ? (hn[0] | (hn[1] << 4)) : 0u);
+        sm.rb[0][lane] = hrbv;
+        hrbv += hrun;
+      }
+      // this wave's share of the next brick must land before the barrier that
+      // ends the chunk; wait now, before the queue's stores are issued
+      __builtin_amdgcn_s_waitcnt(0);
+      lds_barrier();  // every row's cases and prefixes are in LDS
+
+      {
+        int nvl = 0, nfl = 0;
+        uint32_t V = 0, F = 0;
+        auto flush = [&]() {
+          int before = 0;
+          for (uint32_t o0 = 0; o0 < V; o0 += kWave) {
+            const uint32_t o = o0 + lane;
+            const uint64_t word = vbits[o0 >> 6];
+            const uint64_t upto = word & (~0ull >> (kWave - 1 - lane));
+            const int k = before + __popcll(upto) - 1;
+            before += __popcll(word);
+            if (o >= V) break;
+            const uint32_t e = vlist[k];
+            const int lc = (int)(e & 63), c = (int)((e >> 6) & 15);
+            const uint32_t ecs = (e >> 10) & 0xFF, cls = (e >> 18) & 7, sv = e >> 21;
+            const uint32_t q = o - sv;
+            const uint32_t* cell = smp + ((c + 1) * SY + sl) * RD + lc * DW;
+            double px, py, pz;
+            if constexpr (METHOD == MT) {
+              // class-0 voxels (all but the first layers) read the LDS copy
+              uint32_t ed = sm.own0[ecs * 8 + (q & 7)];
+              if (cls != 0) ed = MT_OWN_EDGES[(cls * 256 + nat_to_mt(ecs)) * 20 + q];
+              const uint32_t ew = sm.edge[ed];
+              const T sv0 = *(const T*)(cell + (ew & 0x3FF)), tv = *(const T*)(cell + ((ew >> 10) & 0x3FF));
+              // vertPos (src/marching_tetrahedra.jl:42-55)
+              const M rr = (iso - (M)sv0) / (M)(tv - sv0);
+              const A aw = jl_min<A>(jl_max<A>((A)rr, eps), ome);
+              const A bw = (A)1 - aw;
+              const A wx = (A)((ew >> 20) & 1) * bw + (A)((ew >> 23) & 1) * aw;
+              const A wy = (A)((ew >> 21) & 1) * bw + (A)((ew >> 24) & 1) * aw;
+              const A wz = (A)((ew >> 22) & 1) * bw + (A)((ew >> 25) & 1) * aw;
+              px = mt_axis<A>(cx32, wx, sm.cx[lc], sm.cx[lc + 1]);
+              py = mt_axis<A>(cy32, wy, sm.cy[sl], sm.cy[sl + 1]);
+              pz = mt_axis<A>(cz32, wz, czs[c + 1], czs[c + 2]);
+            } else {
+              // the mean of the crossing points of the cell's 12 edges, each
+              // interpolated as vertex_interp does (src/marching_cubes.jl:100-104),
+              // summed per axis in promote(M, axis type)
+              float fx = 0.f, fy = 0.f, fz = 0.f;
+              double dx = 0.0, dy = 0.0, dz = 0.0;
+              const bool sx = sizeof(M) == 4 && cx32, sy = sizeof(M) == 4 && cy32,
+                         sz = sizeof(M) == 4 && cz32;
+              // the crossing edges in edge order (the order of the sum), one
+              // iteration each rather than a pass over all 12
+              uint32_t xm = sm.xmask[ecs];
+              const int ncross = __popc(xm);
+              while (xm != 0) {
+                const int ed = __builtin_ctz(xm);
+                xm &= xm - 1;
+                const uint32_t ew = sm.edge[ed];
+                const T va = *(const T*)(cell + (ew & 0x3FF)), vb = *(const T*)(cell + ((ew >> 10) & 0x3FF));
+                const M mu = (iso - (M)va) / (M)(vb - va);
+                const double qx = lerp_axis<M>(cx32, mu, sm.cx[lc + ((ew >> 20) & 1)], sm.cx[lc + ((ew >> 23) & 1)]);
+                const double qy = lerp_axis<M>(cy32, mu, sm.cy[sl + ((ew >> 21) & 1)], sm.cy[sl + ((ew >> 24) & 1)]);
+                const double qz = lerp_axis<M>(cz32, mu, czs[c + 1 + ((ew >> 22) & 1)], czs[c + 1 + ((ew >> 25) & 1)]);
+                if (sx) fx = fx + (float)qx; else dx = dx + qx;
+                if (sy) fy = fy + (float)qy; else dy = dy + qy;
+                if (sz) fz = fz + (float)qz; else dz = dz + qz;
+              }
+              px = sx ? (double)(fx / (float)ncross) : dx / (double)ncross;
+              py = sy ? (double)(fy / (float)ncross) : dy / (double)ncross;
+              pz = sz ? (double)(fz / (float)ncross) : dz / (double)ncross;
+            }
+            const int64_t vi = sm.dv[wv][lc] + o;
+            if (vi < totv) {
+              Out* out = verts + vi * 3;
+              out[0] = (Out)px;
+              out[1] = (Out)py;
+              out[2] = (Out)pz;
+            }
+          }
+          before = 0;
+          for (uint32_t o0 = 0; o0 < F; o0 += kWave) {
+            const uint32_t o = o0 + lane;
+            const uint64_t word = fbits[o0 >> 6];
+            const uint64_t upto = word & (~0ull >> (kWave - 1 - lane));
+            const int k = before + __popcll(upto) - 1;
+            before += __popcll(word);
+            if (o >= F) break;
+            const uint32_t e = flist[k];
+            const int lc = (int)(e & 63), c = (int)((e >> 6) & 15);
+            const uint32_t ecs = (e >> 10) & 0xFF, sf = e >> 21;
+            const uint32_t q = o - sf;
+            const int ci = x0 + lc, cj = gj, ck = zc + c;
+            int64_t id[3];
+            if constexpr (METHOD == MT) {
+              const uint32_t fw = sm.face[ecs * 12 + q];
+  #pragma unroll
+              for (int m = 0; m < 3; ++m) {
+                const uint32_t ed = (fw >> (5 * m)) & 31;
+                const uint32_t ew = sm.edge[ed];
+                // the owner is one cell back along each axis on which the edge's
+                // end corner has coordinate 0 (when that cell exists)
+                const int dx = (((ew >> 23) & 1) == 0) & (ci > 0);
+                const int dy = (((ew >> 24) & 1) == 0) & (cj > 0);
+                const int dz = (((ew >> 25) & 1) == 0) & (ck > 0);
+                const uint32_t e2 = sm.shift[ed * 8 + (dx | (dy << 1) | (dz << 2))];
+                const int l2 = lc - dx, w2 = sl - dy, z2 = c - dz;
+                const uint32_t cls2 =
+                    (uint32_t)((ci - dx == 0) | ((cj - dy == 0) << 1) | ((ck - dz == 0) << 2));
+                uint32_t oc;
+                int64_t off;
+                if (z2 >= 0) {
+                  oc = sm.cc[w2][z2][l2];
+                  off = sm.rb[w2][l2] + sm.cp[w2][z2][l2];
+                } else {
+                  oc = sm.pc[w2][l2];
+                  off = sm.rb[w2][l2] - (int64_t)sm.nown[cls2 * 256 + oc];
+                }
+                uint32_t rk = sm.rank0[oc * 19 + e2];
+                if (cls2 != 0) rk = MT_RANK[(cls2 * 256 + nat_to_mt(oc)) * 19 + e2];
+                id[m] = off + (int64_t)rk + 1;
+              }
+            } else {
+              // quads around the crossing edges at corner 0, two triangles each,
+              // axes in x, y, z order; the q-th triangle is triangle q&1 of the
+              // (q>>1)-th quad
+              const uint32_t csm = nat_to_mc(ecs), c0 = csm & 1;
+              const uint32_t axes = (uint32_t)((c0 != ((csm >> 1) & 1)) && cj >= 1 && ck >= 1) |
+                                    ((uint32_t)((c0 != ((csm >> 3) & 1)) && ck >= 1 && ci >= 1) << 1) |
+                                    ((uint32_t)((c0 != ((csm >> 4) & 1)) && ci >= 1 && cj >= 1) << 2);
+              const uint32_t first = axes & (0u - axes);
+              const uint32_t pick = (q >> 1) == 0 ? first
+                                  : (q >> 1) == 1 ? ((axes ^ first) & (0u - (axes ^ first))) : 4u;
+              const int ax = pick == 1 ? 0 : pick == 2 ? 1 : 2;
+              const int u = (ax + 1) % 3, w = (ax + 2) % 3;
+              auto cell_id = [&](int du, int dw) -> int64_t {
+                const int d0 = (u == 0 ? du : 0) + (w == 0 ? dw : 0);
+                const int d1 = (u == 1 ? du : 0) + (w == 1 ? dw : 0);
+                const int d2 = (u == 2 ? du : 0) + (w == 2 ? dw : 0);
+                const int l2 = lc - d0, w2 = sl - d1, z2 = c - d2;
+                if (z2 >= 0) return sm.rb[w2][l2] + sm.cp[w2][z2][l2] + 1;
+                return sm.rb[w2][l2];  // the cell just below the chunk, active: the row's last vertex
+              };
+              const int64_t me = sm.rb[sl][lc] + sm.cp[sl][c][lc] + 1;
+              const int64_t q1 = cell_id(1, 0), q2 = cell_id(1, 1), q3 = cell_id(0, 1);
+              const int64_t a1 = c0 ? q1 : q3, a3 = c0 ? q3 : q1;
+              id[0] = me;
+              id[1] = (q & 1) ? q2 : a1;
+              id[2] = (q & 1) ? a3 : q2;
+            }
+            const int64_t fi = sm.df[wv][lc] + o;
+            if (fi < totf) {
+              int64_t* out = faces + fi * 3;
+              out[0] = id[0];
+              out[1] = id[1];
+              out[2] = id[2];
+            }
+          }
+          wave_sync();
+          for (uint32_t q = lane; q < (V + 63) / 64; q += kWave) vbits[q] = 0;
+          for (uint32_t q = lane; q < (F + 63) / 64; q += kWave) fbits[q] = 0;
+        };
+
+        // queue the emitting cells z in [za, zb) of every row of this wave,
+        // rows one after another, each row's cells in z order
+        auto append = [&](int za, int zb, uint32_t rtot, uint32_t run0) {
+          uint32_t nev = 0, nef = 0;
+  #pragma unroll
+          for (int z = 0; z < ZC; ++z) {
+            const bool in = z >= za && z < zb && emits;
+            nev += (in && nvz[z] != 0) ? 1u : 0u;
+            nef += (in && nfz[z] != 0) ? 1u : 0u;
+          }
+          const uint32_t inct = wave_scan_dpp(rtot), incv = wave_scan_dpp(nev), incf = wave_scan_dpp(nef);
+          const uint32_t sumt = (uint32_t)__builtin_amdgcn_readlane((int)inct, kWave - 1);
+          const uint32_t sumv = (uint32_t)__builtin_amdgcn_readlane((int)incv, kWave - 1);
+          const uint32_t sumf = (uint32_t)__builtin_amdgcn_readlane((int)incf, kWave - 1);
+          const uint32_t bv = V + ((inct - rtot) & 0xFFFF), bf = F + ((inct - rtot) >> 16);
+          uint32_t kv = (uint32_t)nvl + incv - nev, kf = (uint32_t)nfl + incf - nef;
+          if (nev | nef) {
+            sm.dv[wv][lane] = rbv + (int64_t)(run0 & 0xFFFF) - (int64_t)bv;
+            sm.df[wv][lane] = rbf + (int64_t)(run0 >> 16) - (int64_t)bf;
+          }
+          uint32_t run = 0;
+  #pragma unroll
+          for (int z = 0; z < ZC; ++z) {
+            const bool in = z >= za && z < zb && emits;
+            if (in) {
+              const uint32_t cls = (uint32_t)((gi == 0) | ((gj == 0) << 1) | ((zc + z == 0) << 2));
+              const uint32_t head = (uint32_t)lane | ((uint32_t)z << 6) | (cs[z] << 10) | (cls << 18);
+              if (nvz[z] != 0) {
+                const uint32_t sv = bv + (run & 0xFFFF);
+                vlist[kv++] = head | (sv << 21);
+                atomicOr((unsigned long long*)&vbits[sv >> 6], 1ull << (sv & 63));
+              }
+              if (nfz[z] != 0) {
+                const uint32_t sf = bf + (run >> 16);
+                flist[kf++] = head | (sf << 21);
+                atomicOr((unsigned long long*)&fbits[sf >> 6], 1ull << (sf & 63));
+              }
+              run += nvz[z] | (nfz[z] << 16);
+            }
+          }
+          nvl += (int)sumv;
+          nfl += (int)sumf;
+          V += sumt & 0xFFFF;
+          F += sumt >> 16;
+        };
+
+        const uint32_t ttot = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_dpp(tot), kWave - 1);
+        uint32_t cv = 0, cf = 0;
+  #pragma unroll
+        for (int z = 0; z < ZC; ++z) {
+          cv += (emits && nvz[z] != 0) ? 1u : 0u;
+          cf += (emits && nfz[z] != 0) ? 1u : 0u;
+        }
+        const uint32_t tcv = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_dpp(cv), kWave - 1);
+        const uint32_t tcf = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_dpp(cf), kWave - 1);
+        // the whole chunk in one go when it fits the queue, else one z step at a time
+        const bool whole = (ttot & 0xFFFF) <= (uint32_t)CAPV && (ttot >> 16) <= (uint32_t)CAPF &&
+                           tcv <= (uint32_t)CAPE && tcf <= (uint32_t)CAPE;
+        uint32_t run0 = 0;
+        for (int za = 0; ttot != 0 && za < zn;) {
+          const int zb = whole ? ZC : za + 1;
+          uint32_t part = 0;
+  #pragma unroll
+          for (int q = 0; q < ZC; ++q)
+            part += (emits && q >= za && q < zb) ? (nvz[q] | (nfz[q] << 16)) : 0u;
+          append(za, zb, part, run0);
+          run0 += part;
+          wave_sync();
+          flush();
+          wave_sync();
+          nvl = 0;
+          nfl = 0;
+          V = 0;
+          F = 0;
+          za = zb;
+        }
+      }
+      rbv += runv;
+      rbf += tot >> 16;
+      lds_barrier();
     }
   };
-
-  uint32_t* vlist = sm.vlist[wv];
-  uint32_t* flist = sm.flist[wv];
-  uint64_t* vbits = sm.vbits[wv];
-  uint64_t* fbits = sm.fbits[wv];
-  for (int q = lane; q < CAPV / 64; q += kWave) vbits[q] = 0;
-  for (int q = lane; q < CAPF / 64; q += kWave) fbits[q] = 0;
-
-  if (zs < ze) fetch(zs, 0);
-  __builtin_amdgcn_s_waitcnt(0);
-  lds_barrier();
-  int buf = 0;
-  for (int zc = zs; zc < ze; zc += ZC, buf ^= 1) {
-    if (zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
-    const uint32_t* smp = sm.smp[buf];
-    const double* czs = sm.cz[buf];
-
-    // classify this lane's row: natural-order face nibble per plane
-    uint32_t nib[NP];
-#pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      const T v0 = *(const T*)(smp + (p * SY + sl) * RD + lane * DW);
-      const T v1 = *(const T*)(smp + (p * SY + sl + 1) * RD + lane * DW);
-      const uint32_t w = (uint32_t)(v0 < thr) | ((uint32_t)(v1 < thr) << 2);
-      nib[p] = w | (next_lane(w, 0u) << 1);
-    }
-    const int zn = min(ZC, ze - zc);
-    uint32_t cs[ZC], nvz[ZC], nfz[ZC];
-    uint32_t runv = 0, tot = 0;  // in-row vertex prefix; emitted vertices | faces << 16
-#pragma unroll
-    for (int z = 0; z < ZC; ++z) {
-      const int gk = zc + z;
-      cs[z] = (exists && z < zn) ? (nib[z + 1] | (nib[z + 2] << 4)) : 0u;
-      const uint32_t cls = (uint32_t)((gi == 0) | ((gj == 0) << 1) | ((gk == 0) << 2));
-      if constexpr (METHOD == MT) {
-        nvz[z] = sm.nown[cls * 256 + cs[z]];
-        nfz[z] = emits ? (uint32_t)sm.nf[cs[z]] : 0u;
-        sm.cc[sl][z][lane] = (uint8_t)cs[z];
-      } else {
-        const uint32_t csm = nat_to_mc(cs[z]), c0 = csm & 1;
-        nvz[z] = (cs[z] != 0 && cs[z] != 255) ? 1u : 0u;
-        const uint32_t nq = (uint32_t)((c0 != ((csm >> 1) & 1)) && gj >= 1 && gk >= 1) +
-                            (uint32_t)((c0 != ((csm >> 3) & 1)) && gk >= 1 && gi >= 1) +
-                            (uint32_t)((c0 != ((csm >> 4) & 1)) && gi >= 1 && gj >= 1);
-        nfz[z] = (emits && nvz[z]) ? 2u * nq : 0u;
-      }
-      sm.cp[sl][z][lane] = (uint16_t)runv;
-      runv += nvz[z];
-      if (emits) tot += nvz[z] | (nfz[z] << 16);
-    }
-    if constexpr (METHOD == MT) sm.pc[sl][lane] = (uint8_t)((exists && zc > 0) ? (nib[0] | (nib[1] << 4)) : 0u);
-    sm.rb[sl][lane] = rbv;
-    if (wv == 0) {
-      // the row at -y (slot 0): cases and vertex prefixes for the owner
-      // lookups of row jb's faces; it emits nothing here
-      uint32_t hn[NP];
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        const T v0 = *(const T*)(smp + (p * SY) * RD + lane * DW);
-        const T v1 = *(const T*)(smp + (p * SY + 1) * RD + lane * DW);
-        const uint32_t w = (uint32_t)(v0 < thr) | ((uint32_t)(v1 < thr) << 2);
-        hn[p] = w | (next_lane(w, 0u) << 1);
-      }
-      uint32_t hrun = 0;
-#pragma unroll
-      for (int z = 0; z < ZC; ++z) {
-        const uint32_t c = (hexists && z < zn) ? (hn[z + 1] | (hn[z + 2] << 4)) : 0u;
-        uint32_t nv;
-        if constexpr (METHOD == MT) {
-          const uint32_t cls = (uint32_t)((gi == 0) | ((y0 == 0) << 1) | ((zc + z == 0) << 2));
-          nv = sm.nown[cls * 256 + c];
-          sm.cc[0][z][lane] = (uint8_t)c;
-        } else {
-          nv = (c != 0 && c != 255) ? 1u : 0u;
-        }
-        sm.cp[0][z][lane] = (uint16_t)hrun;
-        hrun += nv;
-      }
-      if constexpr (METHOD == MT) sm.pc[0][lane] = (uint8_t)((hexists && zc > 0) ? (hn[0] | (hn[1] << 4)) : 0u);
-      sm.rb[0][lane] = hrbv;
-      hrbv += hrun;
-    }
-    // this wave's share of the next brick must land before the barrier that
-    // ends the chunk; wait now, before the queue's stores are issued
-    __builtin_amdgcn_s_waitcnt(0);
-    lds_barrier();  // every row's cases and prefixes are in LDS
-
-    {
-      int nvl = 0, nfl = 0;
-      uint32_t V = 0, F = 0;
-      auto flush = [&]() {
-        int before = 0;
-        for (uint32_t o0 = 0; o0 < V; o0 += kWave) {
-          const uint32_t o = o0 + lane;
-          const uint64_t word = vbits[o0 >> 6];
-          const uint64_t upto = word & (~0ull >> (kWave - 1 - lane));
-          const int k = before + __popcll(upto) - 1;
-          before += __popcll(word);
-          if (o >= V) break;
-          const uint32_t e = vlist[k];
-          const int lc = (int)(e & 63), c = (int)((e >> 6) & 15);
-          const uint32_t ecs = (e >> 10) & 0xFF, cls = (e >> 18) & 7, sv = e >> 21;
-          const uint32_t q = o - sv;
-          const uint32_t* cell = smp + ((c + 1) * SY + sl) * RD + lc * DW;
-          double px, py, pz;
-          if constexpr (METHOD == MT) {
-            // class-0 voxels (all but the first layers) read the LDS copy
-            uint32_t ed = sm.own0[ecs * 8 + (q & 7)];
-            if (cls != 0) ed = MT_OWN_EDGES[(cls * 256 + nat_to_mt(ecs)) * 20 + q];
-            const uint32_t ew = sm.edge[ed];
-            const T sv0 = *(const T*)(cell + (ew & 0x3FF)), tv = *(const T*)(cell + ((ew >> 10) & 0x3FF));
-            // vertPos (src/marching_tetrahedra.jl:42-55)
-            const M rr = (iso - (M)sv0) / (M)(tv - sv0);
-            const A aw = jl_min<A>(jl_max<A>((A)rr, eps), ome);
-            const A bw = (A)1 - aw;
-            const A wx = (A)((ew >> 20) & 1) * bw + (A)((ew >> 23) & 1) * aw;
-            const A wy = (A)((ew >> 21) & 1) * bw + (A)((ew >> 24) & 1) * aw;
-            const A wz = (A)((ew >> 22) & 1) * bw + (A)((ew >> 25) & 1) * aw;
-            px = mt_axis<A>(cx32, wx, sm.cx[lc], sm.cx[lc + 1]);
-            py = mt_axis<A>(cy32, wy, sm.cy[sl], sm.cy[sl + 1]);
-            pz = mt_axis<A>(cz32, wz, czs[c + 1], czs[c + 2]);
-          } else {
-            // the mean of the crossing points of the cell's 12 edges, each
-            // interpolated as vertex_interp does (src/marching_cubes.jl:100-104),
-            // summed per axis in promote(M, axis type)
-            float fx = 0.f, fy = 0.f, fz = 0.f;
-            double dx = 0.0, dy = 0.0, dz = 0.0;
-            const bool sx = sizeof(M) == 4 && cx32, sy = sizeof(M) == 4 && cy32,
-                       sz = sizeof(M) == 4 && cz32;
-            // the crossing edges in edge order (the order of the sum), one
-            // iteration each rather than a pass over all 12
-            uint32_t xm = sm.xmask[ecs];
-            const int ncross = __popc(xm);
-            while (xm != 0) {
-              const int ed = __builtin_ctz(xm);
-              xm &= xm - 1;
-              const uint32_t ew = sm.edge[ed];
-              const T va = *(const T*)(cell + (ew & 0x3FF)), vb = *(const T*)(cell + ((ew >> 10) & 0x3FF));
-              const M mu = (iso - (M)va) / (M)(vb - va);
-              const double qx = lerp_axis<M>(cx32, mu, sm.cx[lc + ((ew >> 20) & 1)], sm.cx[lc + ((ew >> 23) & 1)]);
-              const double qy = lerp_axis<M>(cy32, mu, sm.cy[sl + ((ew >> 21) & 1)], sm.cy[sl + ((ew >> 24) & 1)]);
-              const double qz = lerp_axis<M>(cz32, mu, czs[c + 1 + ((ew >> 22) & 1)], czs[c + 1 + ((ew >> 25) & 1)]);
-              if (sx) fx = fx + (float)qx; else dx = dx + qx;
-              if (sy) fy = fy + (float)qy; else dy = dy + qy;
-              if (sz) fz = fz + (float)qz; else dz = dz + qz;
-            }
-            px = sx ? (double)(fx / (float)ncross) : dx / (double)ncross;
-            py = sy ? (double)(fy / (float)ncross) : dy / (double)ncross;
-            pz = sz ? (double)(fz / (float)ncross) : dz / (double)ncross;
-          }
-          const int64_t vi = sm.dv[wv][lc] + o;
-          if (vi < totv) {
-            Out* out = verts + vi * 3;
-            out[0] = (Out)px;
-            out[1] = (Out)py;
-            out[2] = (Out)pz;
-          }
-        }
-        before = 0;
-        for (uint32_t o0 = 0; o0 < F; o0 += kWave) {
-          const uint32_t o = o0 + lane;
-          const uint64_t word = fbits[o0 >> 6];
-          const uint64_t upto = word & (~0ull >> (kWave - 1 - lane));
-          const int k = before + __popcll(upto) - 1;
-          before += __popcll(word);
-          if (o >= F) break;
-          const uint32_t e = flist[k];
-          const int lc = (int)(e & 63), c = (int)((e >> 6) & 15);
-          const uint32_t ecs = (e >> 10) & 0xFF, sf = e >> 21;
-          const uint32_t q = o - sf;
-          const int ci = x0 + lc, cj = gj, ck = zc + c;
-          int64_t id[3];
-          if constexpr (METHOD == MT) {
-            const uint32_t fw = sm.face[ecs * 12 + q];
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-              const uint32_t ed = (fw >> (5 * m)) & 31;
-              const uint32_t ew = sm.edge[ed];
-              // the owner is one cell back along each axis on which the edge's
-              // end corner has coordinate 0 (when that cell exists)
-              const int dx = (((ew >> 23) & 1) == 0) & (ci > 0);
-              const int dy = (((ew >> 24) & 1) == 0) & (cj > 0);
-              const int dz = (((ew >> 25) & 1) == 0) & (ck > 0);
-              const uint32_t e2 = sm.shift[ed * 8 + (dx | (dy << 1) | (dz << 2))];
-              const int l2 = lc - dx, w2 = sl - dy, z2 = c - dz;
-              const uint32_t cls2 =
-                  (uint32_t)((ci - dx == 0) | ((cj - dy == 0) << 1) | ((ck - dz == 0) << 2));
-              uint32_t oc;
-              int64_t off;
-              if (z2 >= 0) {
-                oc = sm.cc[w2][z2][l2];
-                off = sm.rb[w2][l2] + sm.cp[w2][z2][l2];
-              } else {
-                oc = sm.pc[w2][l2];
-                off = sm.rb[w2][l2] - (int64_t)sm.nown[cls2 * 256 + oc];
-              }
-              uint32_t rk = sm.rank0[oc * 19 + e2];
-              if (cls2 != 0) rk = MT_RANK[(cls2 * 256 + nat_to_mt(oc)) * 19 + e2];
-              id[m] = off + (int64_t)rk + 1;
-            }
-          } else {
-            // quads around the crossing edges at corner 0, two triangles each,
-            // axes in x, y, z order; the q-th triangle is triangle q&1 of the
-            // (q>>1)-th quad
-            const uint32_t csm = nat_to_mc(ecs), c0 = csm & 1;
-            const uint32_t axes = (uint32_t)((c0 != ((csm >> 1) & 1)) && cj >= 1 && ck >= 1) |
-                                  ((uint32_t)((c0 != ((csm >> 3) & 1)) && ck >= 1 && ci >= 1) << 1) |
-                                  ((uint32_t)((c0 != ((csm >> 4) & 1)) && ci >= 1 && cj >= 1) << 2);
-            const uint32_t first = axes & (0u - axes);
-            const uint32_t pick = (q >> 1) == 0 ? first
-                                : (q >> 1) == 1 ? ((axes ^ first) & (0u - (axes ^ first))) : 4u;
-            const int ax = pick == 1 ? 0 : pick == 2 ? 1 : 2;
-            const int u = (ax + 1) % 3, w = (ax + 2) % 3;
-            auto cell_id = [&](int du, int dw) -> int64_t {
-              const int d0 = (u == 0 ? du : 0) + (w == 0 ? dw : 0);
-              const int d1 = (u == 1 ? du : 0) + (w == 1 ? dw : 0);
-              const int d2 = (u == 2 ? du : 0) + (w == 2 ? dw : 0);
-              const int l2 = lc - d0, w2 = sl - d1, z2 = c - d2;
-              if (z2 >= 0) return sm.rb[w2][l2] + sm.cp[w2][z2][l2] + 1;
-              return sm.rb[w2][l2];  // the cell just below the chunk, active: the row's last vertex
-            };
-            const int64_t me = sm.rb[sl][lc] + sm.cp[sl][c][lc] + 1;
-            const int64_t q1 = cell_id(1, 0), q2 = cell_id(1, 1), q3 = cell_id(0, 1);
-            const int64_t a1 = c0 ? q1 : q3, a3 = c0 ? q3 : q1;
-            id[0] = me;
-            id[1] = (q & 1) ? q2 : a1;
-            id[2] = (q & 1) ? a3 : q2;
-          }
-          const int64_t fi = sm.df[wv][lc] + o;
-          if (fi < totf) {
-            int64_t* out = faces + fi * 3;
-            out[0] = id[0];
-            out[1] = id[1];
-            out[2] = id[2];
-          }
-        }
-        wave_sync();
-        for (uint32_t q = lane; q < (V + 63) / 64; q += kWave) vbits[q] = 0;
-        for (uint32_t q = lane; q < (F + 63) / 64; q += kWave) fbits[q] = 0;
-      };
-
-      // queue the emitting cells z in [za, zb) of every row of this wave,
-      // rows one after another, each row's cells in z order
-      auto append = [&](int za, int zb, uint32_t rtot, uint32_t run0) {
-        uint32_t nev = 0, nef = 0;
-#pragma unroll
-        for (int z = 0; z < ZC; ++z) {
-          const bool in = z >= za && z < zb && emits;
-          nev += (in && nvz[z] != 0) ? 1u : 0u;
-          nef += (in && nfz[z] != 0) ? 1u : 0u;
-        }
-        const uint32_t inct = wave_scan_dpp(rtot), incv = wave_scan_dpp(nev), incf = wave_scan_dpp(nef);
-        const uint32_t sumt = (uint32_t)__builtin_amdgcn_readlane((int)inct, kWave - 1);
-        const uint32_t sumv = (uint32_t)__builtin_amdgcn_readlane((int)incv, kWave - 1);
-        const uint32_t sumf = (uint32_t)__builtin_amdgcn_readlane((int)incf, kWave - 1);
-        const uint32_t bv = V + ((inct - rtot) & 0xFFFF), bf = F + ((inct - rtot) >> 16);
-        uint32_t kv = (uint32_t)nvl + incv - nev, kf = (uint32_t)nfl + incf - nef;
-        if (nev | nef) {
-          sm.dv[wv][lane] = rbv + (int64_t)(run0 & 0xFFFF) - (int64_t)bv;
-          sm.df[wv][lane] = rbf + (int64_t)(run0 >> 16) - (int64_t)bf;
-        }
-        uint32_t run = 0;
-#pragma unroll
-        for (int z = 0; z < ZC; ++z) {
-          const bool in = z >= za && z < zb && emits;
-          if (in) {
-            const uint32_t cls = (uint32_t)((gi == 0) | ((gj == 0) << 1) | ((zc + z == 0) << 2));
-            const uint32_t head = (uint32_t)lane | ((uint32_t)z << 6) | (cs[z] << 10) | (cls << 18);
-            if (nvz[z] != 0) {
-              const uint32_t sv = bv + (run & 0xFFFF);
-              vlist[kv++] = head | (sv << 21);
-              atomicOr((unsigned long long*)&vbits[sv >> 6], 1ull << (sv & 63));
-            }
-            if (nfz[z] != 0) {
-              const uint32_t sf = bf + (run >> 16);
-              flist[kf++] = head | (sf << 21);
-              atomicOr((unsigned long long*)&fbits[sf >> 6], 1ull << (sf & 63));
-            }
-            run += nvz[z] | (nfz[z] << 16);
-          }
-        }
-        nvl += (int)sumv;
-        nfl += (int)sumf;
-        V += sumt & 0xFFFF;
-        F += sumt >> 16;
-      };
-
-      const uint32_t ttot = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_dpp(tot), kWave - 1);
-      uint32_t cv = 0, cf = 0;
-#pragma unroll
-      for (int z = 0; z < ZC; ++z) {
-        cv += (emits && nvz[z] != 0) ? 1u : 0u;
-        cf += (emits && nfz[z] != 0) ? 1u : 0u;
-      }
-      const uint32_t tcv = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_dpp(cv), kWave - 1);
-      const uint32_t tcf = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_dpp(cf), kWave - 1);
-      // the whole chunk in one go when it fits the queue, else one z step at a time
-      const bool whole = (ttot & 0xFFFF) <= (uint32_t)CAPV && (ttot >> 16) <= (uint32_t)CAPF &&
-                         tcv <= (uint32_t)CAPE && tcf <= (uint32_t)CAPE;
-      uint32_t run0 = 0;
-      for (int za = 0; ttot != 0 && za < zn;) {
-        const int zb = whole ? ZC : za + 1;
-        uint32_t part = 0;
-#pragma unroll
-        for (int q = 0; q < ZC; ++q)
-          part += (emits && q >= za && q < zb) ? (nvz[q] | (nfz[q] << 16)) : 0u;
-        append(za, zb, part, run0);
-        run0 += part;
-        wave_sync();
-        flush();
-        wave_sync();
-        nvl = 0;
-        nfl = 0;
-        V = 0;
-        F = 0;
-        za = zb;
-      }
-    }
-    rbv += runv;
-    rbf += tot >> 16;
-    lds_barrier();
+  if constexpr (Src::kOne) {
+    tile(*src.p, src.t);
+  } else {
+    VolArgs a;
+    int64_t t;
+    while (src(a, t)) tile(a, t);
   }
 }
 
 template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM>
 __global__ __launch_bounds__(kBlock) void k_emit_h5(VolArgs a) {
-  emit_h5_tile<METHOD, T, M, A, Out, ZC, VEC, CXM>(a, xcd_tile(blockIdx.x, a.ntiles));
+  emit_h5_run<METHOD, T, M, A, Out, ZC, VEC, CXM>(
+      OneTile{&a, xcd_tile(blockIdx.x, a.ntiles), false});
 }
 
+// Batched, persistent: the launch's blocks take the tiles of its volumes
+// from a work queue (TileQueue).
 template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM>
 __global__ __launch_bounds__(kBlock) void k_emit_h5_b(const VolArgs* __restrict__ vols,
                                                       const int32_t* __restrict__ emap,
-                                                      int64_t gbase) {
-  const int64_t b = gbase + blockIdx.x;
-  const VolArgs a = vols[emap[b]];
-  emit_h5_tile<METHOD, T, M, A, Out, ZC, VEC, CXM>(a, b - a.eblk0);
+                                                      int64_t gbase, int64_t ntiles,
+                                                      uint32_t* __restrict__ ctr) {
+  emit_h5_run<METHOD, T, M, A, Out, ZC, VEC, CXM>(
+      TileQueue<true>{vols, emap, gbase, ntiles, ctr, 0});
 }
 
 }  // namespace
@@ -566,15 +580,13 @@ int h5_variant(const Plan& p, VolArgs* a) {
 
 template <int METHOD, typename T, typename M, typename A, typename Out, int CXM>
 static hipError_t h5_z(bool vec, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
-                       int64_t gbase, int64_t nblocks, hipStream_t st) {
+                       int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st) {
   constexpr int ZC = h5_zc<METHOD, T>();
   if (vols) {
-    if (vec)
-      hipLaunchKernelGGL((k_emit_h5_b<METHOD, T, M, A, Out, ZC, true, CXM>), dim3((unsigned)nblocks),
-                         dim3(kBlock), 0, st, vols, emap, gbase);
-    else
-      hipLaunchKernelGGL((k_emit_h5_b<METHOD, T, M, A, Out, ZC, false, CXM>), dim3((unsigned)nblocks),
-                         dim3(kBlock), 0, st, vols, emap, gbase);
+    const auto k = vec ? k_emit_h5_b<METHOD, T, M, A, Out, ZC, true, CXM>
+                       : k_emit_h5_b<METHOD, T, M, A, Out, ZC, false, CXM>;
+    hipLaunchKernelGGL(k, dim3(persistent_grid((const void*)k, nblocks)), dim3(kBlock), 0, st,
+                       vols, emap, gbase, nblocks, ctr);
   } else {
     if (vec)
       hipLaunchKernelGGL((k_emit_h5<METHOD, T, M, A, Out, ZC, true, CXM>), dim3((unsigned)a.ntiles),
@@ -588,18 +600,19 @@ static hipError_t h5_z(bool vec, const VolArgs& a, const VolArgs* vols, const in
 
 template <int METHOD, typename T, typename M, typename A, typename Out>
 static hipError_t h5_t(int variant, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
-                       int64_t gbase, int64_t nblocks, hipStream_t st) {
+                       int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st) {
   const bool vec = (variant & 1) == 0;
   switch ((variant >> 1) & 3) {
-    case 1: return h5_z<METHOD, T, M, A, Out, 0>(vec, a, vols, emap, gbase, nblocks, st);
-    case 2: return h5_z<METHOD, T, M, A, Out, 7>(vec, a, vols, emap, gbase, nblocks, st);
-    default: return h5_z<METHOD, T, M, A, Out, -1>(vec, a, vols, emap, gbase, nblocks, st);
+    case 1: return h5_z<METHOD, T, M, A, Out, 0>(vec, a, vols, emap, gbase, nblocks, ctr, st);
+    case 2: return h5_z<METHOD, T, M, A, Out, 7>(vec, a, vols, emap, gbase, nblocks, ctr, st);
+    default: return h5_z<METHOD, T, M, A, Out, -1>(vec, a, vols, emap, gbase, nblocks, ctr, st);
   }
 }
 
 static hipError_t h5_any(const Plan& p, int variant, const VolArgs& a, const VolArgs* vols,
-                         const int32_t* emap, int64_t gbase, int64_t nblocks, hipStream_t st) {
-#define MSH_ARGS variant, a, vols, emap, gbase, nblocks, st
+                         const int32_t* emap, int64_t gbase, int64_t nblocks, uint32_t* ctr,
+                         hipStream_t st) {
+#define MSH_ARGS variant, a, vols, emap, gbase, nblocks, ctr, st
   if (p.method == SN) {
     // no eps: A is M
     if (p.T == K_F32 && p.M == K_F32 && p.Out == K_F32)
@@ -619,14 +632,14 @@ static hipError_t h5_any(const Plan& p, int variant, const VolArgs& a, const Vol
 
 hipError_t launch_emit_h5(const Plan& p, int variant, const VolArgs& a, hipStream_t st) {
   if (a.ntiles == 0) return hipSuccess;
-  return h5_any(p, variant, a, nullptr, nullptr, 0, 0, st);
+  return h5_any(p, variant, a, nullptr, nullptr, 0, 0, nullptr, st);
 }
 
 hipError_t launch_emit_h5_batch(const Plan& p, int variant, const VolArgs* vols,
                                 const int32_t* emap, int64_t gbase, int64_t nblocks,
-                                hipStream_t st) {
+                                uint32_t* ctr, hipStream_t st) {
   if (nblocks == 0) return hipSuccess;
-  return h5_any(p, variant, VolArgs{}, vols, emap, gbase, nblocks, st);
+  return h5_any(p, variant, VolArgs{}, vols, emap, gbase, nblocks, ctr, st);
 }
 
 }  // namespace msh

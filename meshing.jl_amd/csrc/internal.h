@@ -124,13 +124,17 @@ hipError_t launch_emit(const Plan& p, const VolArgs& a, hipStream_t st);
 hipError_t launch_coords_batch(const VolArgs* vols, int n, hipStream_t st);
 hipError_t launch_fill_maps(const VolArgs* vols, int n, int32_t* cmap, int32_t* emap,
                             hipStream_t st);
+// Persistent count and emit launches (families 2 and 3) take their tiles
+// from 8 work-queue counters at ctr, which must be zero at launch.
 hipError_t launch_count_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
-                              int64_t gbase, int64_t nblocks, hipStream_t st);
+                              int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st);
 hipError_t launch_sizes_batch(const VolArgs* vols, int n, const Workspace& w, int64_t units,
                               int64_t* sizes, hipStream_t st);
 hipError_t launch_emit_batch(const Plan& p, int variant, const VolArgs* vols,
-                             const int32_t* emap, int64_t gbase, int64_t nblocks,
+                             const int32_t* emap, int64_t gbase, int64_t nblocks, uint32_t* ctr,
                              hipStream_t st);
+// Zero n 32-bit words (the work-queue counters) with a kernel.
+hipError_t launch_zero_u32(uint32_t* p, int64_t n, hipStream_t st);
 // Exact-coordinate vertex welding (weld.hip).
 struct WeldArgs {
   int out_kind;          // K_F32 or K_F64: element type of the coordinates

@@ -20,6 +20,10 @@
 #include <stdint.h>
 #include <stdlib.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "internal.h"
 #include "device.h"
 
@@ -114,6 +118,11 @@ __device__ __forceinline__ void st_relaxed(int64_t* p, int64_t v) {
 }
 
 __global__ void k_zero_u64(uint64_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
@@ -901,17 +910,18 @@ static hipError_t emit_types(const Plan& p, int sh, const VolArgs& a, const VolA
 }
 
 static hipError_t emit_any(const Plan& p, int variant, const VolArgs& a, const VolArgs* vols,
-                           const int32_t* emap, int64_t gbase, int64_t nblocks, hipStream_t st) {
+                           const int32_t* emap, int64_t gbase, int64_t nblocks, uint32_t* ctr,
+                           hipStream_t st) {
   if (variant < 32) {
     if (vols) return launch_emit_mc4_batch(p, variant, vols, emap, gbase, nblocks, st);
     return launch_emit_mc4(p, variant, a, st);
   }
   if (variant >= 192) {
-    if (vols) return launch_emit_h5_batch(p, variant - 192, vols, emap, gbase, nblocks, st);
+    if (vols) return launch_emit_h5_batch(p, variant - 192, vols, emap, gbase, nblocks, ctr, st);
     return launch_emit_h5(p, variant - 192, a, st);
   }
   if (variant >= 128) {
-    if (vols) return launch_emit_mc5_batch(p, variant - 128, vols, emap, gbase, nblocks, st);
+    if (vols) return launch_emit_mc5_batch(p, variant - 128, vols, emap, gbase, nblocks, ctr, st);
     return launch_emit_mc5(p, variant - 128, a, st);
   }
   if (variant >= 64) {
@@ -931,7 +941,7 @@ static hipError_t emit_any(const Plan& p, int variant, const VolArgs& a, const V
 hipError_t launch_emit(const Plan& p, const VolArgs& a0, hipStream_t st) {
   VolArgs a = a0;
   const int variant = emit_variant(p, &a);
-  return emit_any(p, variant, a, nullptr, nullptr, 0, 0, st);
+  return emit_any(p, variant, a, nullptr, nullptr, 0, 0, nullptr, st);
 }
 
 hipError_t launch_coords_batch(const VolArgs* vols, int n, hipStream_t st) {
@@ -948,8 +958,38 @@ hipError_t launch_fill_maps(const VolArgs* vols, int n, int32_t* cmap, int32_t* 
 }
 
 hipError_t launch_count_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
-                              int64_t gbase, int64_t nblocks, hipStream_t st) {
-  return launch_count4_batch(p, vols, cmap, gbase, nblocks, st);
+                              int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st) {
+  return launch_count4_batch(p, vols, cmap, gbase, nblocks, ctr, st);
+}
+
+hipError_t launch_zero_u32(uint32_t* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+unsigned persistent_grid(const void* kernel, int64_t ntiles) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, int64_t> cache;  // (device, kernel) -> blocks
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int64_t resident = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({dev, kernel});
+    if (it != cache.end()) {
+      resident = it->second;
+    } else {
+      int per_cu = 0, cus = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess)
+        per_cu = 1;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+      resident = (int64_t)(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+      cache[{dev, kernel}] = resident;
+    }
+  }
+  return (unsigned)(ntiles < resident ? ntiles : resident);
 }
 
 hipError_t launch_sizes_batch(const VolArgs* vols, int n, const Workspace& w, int64_t units,
@@ -961,9 +1001,9 @@ hipError_t launch_sizes_batch(const VolArgs* vols, int n, const Workspace& w, in
 }
 
 hipError_t launch_emit_batch(const Plan& p, int variant, const VolArgs* vols,
-                             const int32_t* emap, int64_t gbase, int64_t nblocks,
+                             const int32_t* emap, int64_t gbase, int64_t nblocks, uint32_t* ctr,
                              hipStream_t st) {
-  return emit_any(p, variant, VolArgs{}, vols, emap, gbase, nblocks, st);
+  return emit_any(p, variant, VolArgs{}, vols, emap, gbase, nblocks, ctr, st);
 }
 
 hipError_t launch_generate(int kind, int dtype, int64_t nx, int64_t ny, int64_t nz,

@@ -43,13 +43,13 @@ __device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, 
 // even positions, and ORs in the x+1 column's bits from the next lane at
 // odd positions; cell row r's face nibble is then bits [2r, 2r+4).
 template <typename T, int METHOD, int TY, int ZU>
-__device__ __forceinline__ void count4_tile(const VolArgs& a, int64_t t) {
-  static_assert(2 * (TY + 1) <= 32, "mask must fit 32 bits");
-  const T* __restrict__ f = (const T*)a.f;
-  const Geom g = a.g;
-  const T thr = (T)a.thr;
-  uint32_t* __restrict__ cnt = a.cnt + a.ubase;
-  const int nbx = a.cnbx, nby = a.cnby;
+__device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
+                                            const uint32_t* __restrict__ lut);
+
+// The count kernels' tile loop: the per-cell count table goes to LDS once
+// per block, then the tiles of `src` are counted one after another.
+template <typename T, int METHOD, int TY, int ZU, class Src>
+__device__ __forceinline__ void count4_run(Src src) {
   constexpr int NL = METHOD == MC ? 256 : 2048;
   __shared__ uint32_t lut[NL];  // vertices | faces << 16, by class << 8 | natural case
   for (int c = threadIdx.x; c < NL; c += kBlock) {
@@ -78,7 +78,24 @@ __device__ __forceinline__ void count4_tile(const VolArgs& a, int64_t t) {
     lut[c] = v;
   }
   __syncthreads();
+  if constexpr (Src::kOne) {
+    count4_body<T, METHOD, TY, ZU>(*src.p, src.t, lut);
+  } else {
+    VolArgs a;
+    int64_t t;
+    while (src(a, t)) count4_body<T, METHOD, TY, ZU>(a, t, lut);
+  }
+}
 
+template <typename T, int METHOD, int TY, int ZU>
+__device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
+                                            const uint32_t* __restrict__ lut) {
+  static_assert(2 * (TY + 1) <= 32, "mask must fit 32 bits");
+  const T* __restrict__ f = (const T*)a.f;
+  const Geom g = a.g;
+  const T thr = (T)a.thr;
+  uint32_t* __restrict__ cnt = a.cnt + a.ubase;
+  const int nbx = a.cnbx, nby = a.cnby;
   const int lane = lane_id(), wv = wave_id();
   const int by = (int)(t % nby);
   const int64_t rest = t / nby;
@@ -176,18 +193,18 @@ __device__ __forceinline__ void count4_tile(const VolArgs& a, int64_t t) {
 
 template <typename T, int METHOD, int TY, int ZU>
 __global__ __launch_bounds__(kBlock) void k_count4(VolArgs a) {
-  count4_tile<T, METHOD, TY, ZU>(a, xcd_tile(blockIdx.x, a.cntiles));
+  count4_run<T, METHOD, TY, ZU>(OneTile{&a, xcd_tile(blockIdx.x, a.cntiles), false});
 }
 
-// Batched: block b of the launch is block gbase + b of the batch; the map
-// names its volume, whose tiles start at block cblk0.
+// Batched, persistent: the launch's blocks take count tiles (block gbase + i
+// of the batch; the map names its volume, whose tiles start at block cblk0)
+// from a work queue.
 template <typename T, int METHOD, int TY, int ZU>
 __global__ __launch_bounds__(kBlock) void k_count4_b(const VolArgs* __restrict__ vols,
                                                      const int32_t* __restrict__ cmap,
-                                                     int64_t gbase) {
-  const int64_t b = gbase + blockIdx.x;
-  const VolArgs a = vols[cmap[b]];
-  count4_tile<T, METHOD, TY, ZU>(a, b - a.cblk0);
+                                                     int64_t gbase, int64_t ntiles,
+                                                     uint32_t* __restrict__ ctr) {
+  count4_run<T, METHOD, TY, ZU>(TileQueue<false>{vols, cmap, gbase, ntiles, ctr, 0});
 }
 
 // ---------------------------------------------------------------------------
@@ -536,39 +553,42 @@ void count4_tiling(const Plan& p, VolArgs* a) {
 
 template <typename T, int METHOD>
 static hipError_t count4_t(const VolArgs& a, const VolArgs* vols, const int32_t* cmap,
-                           int64_t gbase, int64_t nblocks, hipStream_t st) {
+                           int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st) {
   constexpr int TY = count_ty<T>();
-  if (vols)
-    hipLaunchKernelGGL((k_count4_b<T, METHOD, TY, count_zu<T>()>), dim3((unsigned)nblocks),
-                       dim3(kBlock), 0, st, vols, cmap, gbase);
-  else
+  if (vols) {
+    const auto k = k_count4_b<T, METHOD, TY, count_zu<T>()>;
+    hipLaunchKernelGGL(k, dim3(persistent_grid((const void*)k, nblocks)), dim3(kBlock), 0, st,
+                       vols, cmap, gbase, nblocks, ctr);
+  } else
     hipLaunchKernelGGL((k_count4<T, METHOD, TY, count_zu<T>()>), dim3((unsigned)a.cntiles),
                        dim3(kBlock), 0, st, a);
   return hipGetLastError();
 }
 
 static hipError_t count4_any(const Plan& p, const VolArgs& a, const VolArgs* vols,
-                             const int32_t* cmap, int64_t gbase, int64_t nblocks,
+                             const int32_t* cmap, int64_t gbase, int64_t nblocks, uint32_t* ctr,
                              hipStream_t st) {
+#define MSH_ARGS a, vols, cmap, gbase, nblocks, ctr, st
   if (p.T == K_F32) {
-    if (p.method == MC) return count4_t<float, MC>(a, vols, cmap, gbase, nblocks, st);
-    if (p.method == MT) return count4_t<float, MT>(a, vols, cmap, gbase, nblocks, st);
-    return count4_t<float, SN>(a, vols, cmap, gbase, nblocks, st);
+    if (p.method == MC) return count4_t<float, MC>(MSH_ARGS);
+    if (p.method == MT) return count4_t<float, MT>(MSH_ARGS);
+    return count4_t<float, SN>(MSH_ARGS);
   }
-  if (p.method == MC) return count4_t<double, MC>(a, vols, cmap, gbase, nblocks, st);
-  if (p.method == MT) return count4_t<double, MT>(a, vols, cmap, gbase, nblocks, st);
-  return count4_t<double, SN>(a, vols, cmap, gbase, nblocks, st);
+  if (p.method == MC) return count4_t<double, MC>(MSH_ARGS);
+  if (p.method == MT) return count4_t<double, MT>(MSH_ARGS);
+  return count4_t<double, SN>(MSH_ARGS);
+#undef MSH_ARGS
 }
 
 hipError_t launch_count4(const Plan& p, const VolArgs& a, hipStream_t st) {
   if (a.cntiles == 0) return hipSuccess;
-  return count4_any(p, a, nullptr, nullptr, 0, 0, st);
+  return count4_any(p, a, nullptr, nullptr, 0, 0, nullptr, st);
 }
 
 hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
-                               int64_t gbase, int64_t nblocks, hipStream_t st) {
+                               int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st) {
   if (nblocks == 0) return hipSuccess;
-  return count4_any(p, VolArgs{}, vols, cmap, gbase, nblocks, st);
+  return count4_any(p, VolArgs{}, vols, cmap, gbase, nblocks, ctr, st);
 }
 
 // Tile shape of the Marching Cubes emit: z chunks of 16 cells and one cell

@@ -103,3 +103,63 @@ def test_batches_in_a_row_and_state(pkg, oracle):
 
 def test_empty_batch(pkg):
     assert pkg.isosurface_batch([]) == []
+
+
+def _graph_launches(pkg):
+    import ctypes
+    from importlib import import_module
+    capi = import_module(pkg.__name__ + "._capi")
+    ctx = capi.context(0)
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    with ctx.lock:
+        ctx.check(capi.lib().msh_batch_graph_launches(ctx.handle, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def test_repeated_batch_replays_graphs(pkg):
+    # A batch that repeats runs from captured hipGraphs (second call
+    # captures, later calls replay).  Every replay, with eager library work
+    # and synchronisations in between, must equal the single-volume results;
+    # a changed batch must not reuse the graphs.
+    import torch
+    rng = np.random.default_rng(11)
+    kinds = ["sphere", "torus", "gyroid", "blobs", "value_noise"]
+    fields, methods = [], []
+    for v in range(14):
+        shape = tuple(int(s) for s in rng.integers(20, 90, size=3))
+        fields.append(pkg.generate_field(kinds[v % 5], shape, "float64" if v % 4 == 0 else "float32",
+                                         seed=v))
+        methods.append([pkg.MarchingCubes(iso=0), pkg.MarchingTetrahedra(iso=0),
+                        pkg.NaiveSurfaceNets(iso=0)][v % 3])
+    want = [pkg.isosurface(f, m) for f, m in zip(fields, methods)]
+    other = pkg.generate_field("gyroid", (50, 40, 30), "float32")
+    other_want = pkg.isosurface(other, pkg.MarchingTetrahedra(iso=0))
+    torch.cuda.synchronize()
+    c0, e0 = _graph_launches(pkg)
+    s = torch.cuda.Stream()  # graphs are captured on a stream other than the null stream
+    for rep in range(6):
+        with torch.cuda.stream(s):
+            res = pkg.isosurface_batch(fields, methods)
+        s.synchronize()
+        for (V, F), (Vs, Fs) in zip(res, want):
+            assert torch.equal(F, Fs), rep
+            ib = torch.int32 if V.dtype == torch.float32 else torch.int64
+            assert torch.equal(V.view(ib), Vs.view(ib)), rep
+        # eager work between replays: a single extraction and torch ops
+        V2, F2 = pkg.isosurface(other, pkg.MarchingTetrahedra(iso=0))
+        assert torch.equal(F2, other_want[1])
+        del res
+        torch.cuda.synchronize()
+    c1, e1 = _graph_launches(pkg)
+    assert c1 - c0 >= 4 and e1 - e0 >= 4, (c1 - c0, e1 - e0)
+    # a different batch (one field changed) runs eagerly and correctly
+    fields2 = list(fields)
+    fields2[3] = pkg.generate_field("sphere", (33, 34, 35), "float32")
+    want2 = pkg.isosurface(fields2[3], methods[3])
+    with torch.cuda.stream(s):
+        res2 = pkg.isosurface_batch(fields2, methods)
+    s.synchronize()
+    assert torch.equal(res2[3][1], want2[1])
+    assert torch.equal(res2[0][1], want[0][1])
+    c2, e2 = _graph_launches(pkg)
+    assert (c2, e2) == (c1, e1)
