@@ -232,6 +232,16 @@ def measure_batch(m, nvol, steps, seed=7):
     import torch
     fields, methods = batch_workload(m, nvol, seed)
     torch.cuda.synchronize()
+    # kernel families chosen per (method, element type, size class) by
+    # timing the batch's own groups as batched launches; the plain loop then
+    # uses the same choices
+    chosen = m.autotune_batch(fields, methods)
+    tuned = {}
+    for f, me, fam in zip(fields, methods, chosen):
+        cells = (f.shape[0] - 1) * (f.shape[1] - 1) * (f.shape[2] - 1)
+        tuned[f"{['mc', 'mt', 'sn'][me.method]}/"
+              f"{0 if cells < 2 ** 20 else 1 if cells < 2 ** 23 else 2}"] = m.KERNEL_FAMILIES[fam]
+    tuned = dict(sorted(tuned.items()))
     plain = [Runner(m, f, me) for f, me in zip(fields, methods)]
     br = BatchRunner(m, fields, methods)
     for v in range(nvol):
@@ -261,6 +271,7 @@ def measure_batch(m, nvol, steps, seed=7):
                                                ctypes.byref(g[1])))
     res = dict(volumes=nvol, samples=samples, seed=seed,
                graph_launches={"count_phase": g[0].value, "emit_phase": g[1].value},
+               autotuned_emit_kernels=tuned,
                faces=sum(br.nf[v] for v in range(nvol)),
                plain_ms=round(tp * 1e3, 3), batched_ms=round(tb * 1e3, 3),
                plain_req_s=round(nvol / tp, 1), batched_req_s=round(nvol / tb, 1),

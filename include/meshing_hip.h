@@ -122,19 +122,33 @@ int msh_batch_graph_launches(void* ctx, int64_t* count_phase, int64_t* emit_phas
 
 /* Kernel choice.  The emit pass has kernel families per method: 2 lanes
  * along x (the default), 1 lanes along z, 0 a generic kernel, and 3 the
- * lanes-along-x kernel with half-depth bricks (a tile shape; Marching Cubes
- * with one element type throughout and Float64 axes, otherwise the same as
- * 2); each family falls back to the next one down for volumes it cannot
- * take.
+ * lanes-along-x kernel with half-depth bricks (a tile shape, for Naive
+ * Surface Nets and for Marching Cubes with one element type throughout and
+ * Float64 axes; otherwise the same as 2); each family falls back to the
+ * next one down for volumes it cannot take.
  * msh_autotune counts the given field, times the emit of every family that
  * takes it (the best of 3 to 50 launches each, about 2 ms, on the
  * context's stream, into a scratch buffer) and records the fastest for the
- * field's (method, element type) in the context, where later msh_count and
- * msh_batch_count calls use it;
- * *family returns the choice.  msh_set_kernel_family sets it directly (-1
- * restores the default).  Results do not depend on the family.  No
- * counterpart in the reference (its tile shapes are the CPU loop). */
+ * field's (method, element type, size class) in the context, where later
+ * msh_count and msh_batch_count calls on volumes of that class use it (size
+ * classes by cell count: below 2^20, below 2^23, larger; a batch of mixed
+ * sizes then runs each volume with its class's choice);
+ * *family returns the choice.  msh_set_kernel_family sets it directly for
+ * every size class (-1 restores the default).  Results do not depend on the
+ * family.  No counterpart in the reference (its tile shapes are the CPU
+ * loop). */
 int msh_autotune(void* ctx, const msh_field* field, const msh_params* params, int32_t* family);
+/* msh_autotune for a batch: the batch's volumes are grouped by (method,
+ * element type, size class), and each group's batched emit (all its volumes
+ * in one msh_batch_extract, device outputs in a scratch buffer) is timed
+ * with every family that takes all of them (a warm-up, then the best of 3);
+ * the fastest is recorded for the group, and chosen[v] (n entries, may be
+ * null) returns volume v's.  A batch runs the persistent batched kernels,
+ * whose best shape differs from a lone small volume's (which cannot fill
+ * the device by itself).  Leaves no batch state behind: call
+ * msh_batch_count next. */
+int msh_autotune_batch(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
+                       int32_t* chosen);
 int msh_set_kernel_family(void* ctx, int32_t method, int32_t dtype, int32_t family);
 
 /* Vertex welding, the opt-in step after extraction: vertices whose three

@@ -34,7 +34,7 @@ from ._capi import (MSH_DEVICE, MSH_FLOAT32, MSH_FLOAT64, MSH_HOST, MSH_INT64, M
 
 __all__ = ["isosurface", "isosurface_batch", "AbstractMeshingAlgorithm", "MarchingCubes",
            "MarchingTetrahedra", "NaiveSurfaceNets", "device_field", "generate_field", "count",
-           "autotune", "set_kernel_family", "KERNEL_FAMILIES", "Server", "FieldSpec", "weld",
+           "autotune", "autotune_batch", "set_kernel_family", "KERNEL_FAMILIES", "Server", "FieldSpec", "weld",
            "read_nrrd", "write_nrrd", "NrrdInfo", "read_ply", "write_ply"]
 
 # emit kernel families, best first (see msh_autotune in include/meshing_hip.h)
@@ -339,10 +339,12 @@ def isosurface_batch(fields, methods=None, axes=None, device=0):
 
 def autotune(sdf, method=None, *args, X=(-1, 1), Y=(-1, 1), Z=(-1, 1), device=0):
     """Time every emit kernel family on this field and method and keep the
-    fastest for its (method, element type) in the device's context; later
-    calls of isosurface, isosurface_batch and the server on that device use
-    it.  Returns the chosen family (3 lanes along x with half-depth bricks,
-    2 lanes along x, 1 lanes along z, 0 generic; see KERNEL_FAMILIES)."""
+    fastest for its (method, element type, size class) in the device's
+    context (size classes by cell count: below 2^20, below 2^23, larger);
+    later calls of isosurface, isosurface_batch and the server on that
+    device use it for volumes of that class.  Returns the chosen family (3
+    lanes along x with half-depth bricks, 2 lanes along x, 1 lanes along z,
+    0 generic; see KERNEL_FAMILIES)."""
     method, X, Y, Z = _split_args(method, args, X, Y, Z)
     f, keep, on_dev = _field(sdf)
     p = _params(method, X, Y, Z)
@@ -357,9 +359,38 @@ def autotune(sdf, method=None, *args, X=(-1, 1), Y=(-1, 1), Z=(-1, 1), device=0)
     return fam.value
 
 
+def autotune_batch(fields, methods=None, axes=None):
+    """:func:`autotune` for a batch of device fields: each (method, element
+    type, size class) group of the batch is timed as one batched emit with
+    every kernel family, and the fastest is kept for that group; later
+    batches (and single calls) on the device use it.  Returns the family
+    chosen for each volume."""
+    n = len(fields)
+    if methods is None or isinstance(methods, AbstractMeshingAlgorithm):
+        methods = [methods or MarchingCubes()] * n
+    if axes is None:
+        axes = [None] * n
+    if n == 0:
+        return []
+    prepared = [_field(f) for f in fields]
+    if not all(d for _, _, d in prepared) or len({k.device for _, k, _ in prepared}) != 1:
+        raise ValueError("autotune_batch takes CUDA tensors on one device")
+    dev = prepared[0][1].device
+    F_arr = (MshField * n)(*[f for f, _, _ in prepared])
+    P_arr = (MshParams * n)(*[_params(m, *(ax if ax is not None else ((-1, 1),) * 3))
+                             for m, ax in zip(methods, axes)])
+    chosen = (ctypes.c_int32 * n)()
+    ctx = context(dev.index)
+    with ctx.lock:
+        ctx.check(_capi.lib().msh_set_stream(
+            ctx.handle, ctypes.c_void_p(_torch().cuda.current_stream(dev).cuda_stream)))
+        ctx.check(_capi.lib().msh_autotune_batch(ctx.handle, n, F_arr, P_arr, chosen))
+    return list(chosen)
+
+
 def set_kernel_family(method, dtype, family, device=0):
-    """Choose the emit kernel family for (method, element type) on a device
-    (-1: default order).  Results do not depend on it."""
+    """Choose the emit kernel family for (method, element type) on a device,
+    for every size class (-1: default order).  Results do not depend on it."""
     m = method.method if isinstance(method, AbstractMeshingAlgorithm) else int(method)
     name = getattr(dtype, "__name__", None) or str(dtype).replace("torch.", "")
     kd = MSH_FLOAT32 if name == "float32" else MSH_FLOAT64

@@ -21,7 +21,7 @@ EXPORTS = ("msh_create", "msh_destroy", "msh_set_stream", "msh_synchronize",
            "msh_last_error", "msh_output_kind", "msh_count", "msh_extract",
            "msh_batch_count", "msh_batch_extract", "msh_autotune", "msh_set_kernel_family",
            "msh_generate_field", "msh_weld", "msh_isosurface_noalloc",
-           "msh_batch_graph_launches")
+           "msh_batch_graph_launches", "msh_autotune_batch")
 
 
 class MshField(ctypes.Structure):
@@ -98,6 +98,7 @@ def lib():
             L.msh_weld.argtypes = [vp, i32, i64, vp, i64, vp, vp, ctypes.POINTER(i64)]
             L.msh_isosurface_noalloc.argtypes = [vp, pf, pp, vp, i64, vp, i64, vp]
             L.msh_batch_graph_launches.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+            L.msh_autotune_batch.argtypes = [vp, i32, pf, pp, ctypes.POINTER(i32)]
             for name in EXPORTS:
                 f = getattr(L, name)
                 if name != "msh_last_error":

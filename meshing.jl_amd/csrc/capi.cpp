@@ -80,9 +80,14 @@ struct Ctx {
   void* scratch = nullptr;    // this device's static scratch (msh_isosurface_noalloc)
   void* wws = nullptr;        // weld: hash table, scan workspace, indices
   size_t wws_bytes = 0;
-  // emit kernel family per (method, element type), -1 = default order; set
-  // by msh_autotune or msh_set_kernel_family
-  int family[3][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};
+  // emit kernel family per (method, element type, size class), -1 = default
+  // order; set by msh_autotune or msh_set_kernel_family
+  int family[3][3][kSizeClasses];
+  Ctx() {
+    for (auto& m : family)
+      for (auto& t : m)
+        for (int& f : t) f = -1;
+  }
   // state of the last successful msh_count
   bool have = false;
   msh_field field{};
@@ -485,7 +490,7 @@ int msh_count(void* ctx, const msh_field* f, const msh_params* p, int64_t* nvert
   Plan pl;
   int rc = make_plan(c, f, p, &pl);
   if (rc) return rc;
-  pl.family = c->family[pl.method][pl.T];
+  pl.family = c->family[pl.method][pl.T][size_class(f)];
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
   Geom g = make_geom(f);
   Workspace w{};
@@ -578,7 +583,7 @@ int msh_isosurface_noalloc(void* ctx, const msh_field* f, const msh_params* p, v
   Plan pl;
   int rc = make_plan(c, f, p, &pl);
   if (rc) return rc;
-  pl.family = c->family[pl.method][pl.T];
+  pl.family = c->family[pl.method][pl.T][size_class(f)];
   if (f->memory != MSH_DEVICE)
     return fail(c, MSH_ERR_ARGUMENT, "msh_isosurface_noalloc takes a device field");
   if (vcap < 0 || fcap < 0) return fail(c, MSH_ERR_ARGUMENT, "negative output capacity");
@@ -636,7 +641,7 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
       c->err = "volume " + std::to_string(v) + ": " + c->err;
       return rc;
     }
-    b.plans[v].family = c->family[b.plans[v].method][b.plans[v].T];
+    b.plans[v].family = c->family[b.plans[v].method][b.plans[v].T][size_class(&fields[v])];
     geoms[v] = make_geom(&fields[v], true);
     if (geoms[v].units > 0 && !fields[v].data)
       return fail(c, MSH_ERR_ARGUMENT, "volume " + std::to_string(v) + ": null field data");
@@ -792,7 +797,7 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
   std::vector<char> key;
   key_add(&key, fields, (size_t)n);
   key_add(&key, params, (size_t)n);
-  key_add(&key, &c->family[0][0], 9);
+  key_add(&key, &c->family[0][0][0], 9 * kSizeClasses);
   const void* bufs[5] = {c->ws, c->bws, c->hbatch, c->fstage, (const void*)c->stream};
   key_add(&key, bufs, 5);
   hipError_t e = run_phase(c, &c->gcount, std::move(key), !host_fields && c->stream != nullptr,
@@ -880,7 +885,7 @@ int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_p
   std::vector<char> key;
   key_add(&key, b.fields.data(), (size_t)n);
   key_add(&key, b.params.data(), (size_t)n);
-  key_add(&key, &c->family[0][0], 9);
+  key_add(&key, &c->family[0][0][0], 9 * kSizeClasses);
   const void* bufs[5] = {c->ws, c->bws, c->hbatch, (const void*)c->stream,
                          (const void*)c->upload};
   key_add(&key, bufs, 5);
@@ -914,7 +919,7 @@ int msh_set_kernel_family(void* ctx, int32_t method, int32_t dtype, int32_t fami
   if (method < 0 || method > 2 || (dtype != K_F32 && dtype != K_F64) || family < -1 ||
       family > 3)
     return fail(c, MSH_ERR_ARGUMENT, "method, dtype or kernel family out of range");
-  c->family[method][dtype] = family;
+  for (int& f : c->family[method][dtype]) f = family;
   c->have = false;
   c->batch.have = false;
   return MSH_OK;
@@ -927,7 +932,8 @@ int msh_autotune(void* ctx, const msh_field* f, const msh_params* p, int32_t* fa
   int rc = msh_count(ctx, f, p, &nv, &nf);
   if (rc) return rc;
   const Plan base = c->plan;
-  if (family) *family = c->family[base.method][base.T];
+  const int cls = size_class(f);
+  if (family) *family = c->family[base.method][base.T][cls];
   if (nv == 0 && nf == 0) return MSH_OK;  // nothing to time
   const size_t osz = base.Out == K_F32 ? 4 : 8;
   const size_t vbytes = align256((size_t)nv * 3 * osz), fbytes = (size_t)nf * 3 * 8;
@@ -969,9 +975,109 @@ int msh_autotune(void* ctx, const msh_field* f, const msh_params* p, int32_t* fa
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (e != hipSuccess) return hip_fail(c, e, "autotune");
-  c->family[base.method][base.T] = best;
+  c->family[base.method][base.T][cls] = best;
   c->plan.family = best;
   if (family) *family = best;
+  return MSH_OK;
+}
+
+int msh_autotune_batch(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
+                       int32_t* chosen) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return MSH_ERR_ARGUMENT;
+  if (n < 0 || (n > 0 && (!fields || !params)))
+    return fail(c, MSH_ERR_ARGUMENT, "null fields or params");
+  // the groups that choose together: (method, element type, size class)
+  std::vector<int> group((size_t)n);
+  std::vector<std::vector<int>> members;
+  std::vector<int> gkey;
+  for (int v = 0; v < n; ++v) {
+    Plan pl;
+    int rc = make_plan(c, &fields[v], &params[v], &pl);
+    if (rc) return rc;
+    const int k = (pl.method * 3 + pl.T) * kSizeClasses + size_class(&fields[v]);
+    size_t g = 0;
+    while (g < gkey.size() && gkey[g] != k) ++g;
+    if (g == gkey.size()) {
+      gkey.push_back(k);
+      members.emplace_back();
+    }
+    members[g].push_back(v);
+    group[v] = (int)g;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int rc = e == hipSuccess ? MSH_OK : hip_fail(c, e, "autotune events");
+  std::vector<int> best((size_t)gkey.size(), -1);
+  for (size_t g = 0; g < gkey.size() && rc == MSH_OK; ++g) {
+    const int method = gkey[g] / (3 * kSizeClasses), T = (gkey[g] / kSizeClasses) % 3,
+              cls = gkey[g] % kSizeClasses;
+    const size_t m = members[g].size();
+    std::vector<msh_field> fs(m);
+    std::vector<msh_params> ps(m);
+    for (size_t i = 0; i < m; ++i) {
+      fs[i] = fields[members[g][i]];
+      ps[i] = params[members[g][i]];
+    }
+    float best_ms = 0.f;
+    for (int fam = 3; fam >= 0 && rc == MSH_OK; --fam) {
+      c->family[method][T][cls] = fam;
+      std::vector<int64_t> nv(m), nf(m);
+      rc = msh_batch_count(ctx, (int32_t)m, fs.data(), ps.data(), nv.data(), nf.data());
+      if (rc) break;
+      // every volume of the group must take this family (else it is the
+      // same launch as a lower family's)
+      bool takes = true;
+      for (size_t i = 0; i < m && takes; ++i) {
+        Plan q = c->batch.plans[i];
+        VolArgs probe = c->batch.vols[i];
+        takes = variant_family(emit_variant(q, &probe)) == fam;
+      }
+      if (!takes) continue;
+      size_t total = 256;
+      for (size_t i = 0; i < m; ++i)
+        total += align256((size_t)nv[i] * 3 * (c->batch.plans[i].Out == K_F32 ? 4 : 8)) +
+                 align256((size_t)nf[i] * 24);
+      rc = grow(c, &c->ostage, &c->ostage_bytes, total, "autotune output buffer");
+      if (rc) break;
+      std::vector<void*> vp(m);
+      std::vector<int64_t*> fp(m);
+      size_t off = 0;
+      for (size_t i = 0; i < m; ++i) {
+        vp[i] = (char*)c->ostage + off;
+        off += align256((size_t)nv[i] * 3 * (c->batch.plans[i].Out == K_F32 ? 4 : 8));
+        fp[i] = (int64_t*)((char*)c->ostage + off);
+        off += align256((size_t)nf[i] * 24);
+      }
+      float ms = 0.f;
+      for (int rep = 0; rep < 4 && rc == MSH_OK; ++rep) {  // a warm-up, then the best of 3
+        e = hipEventRecord(e0, c->stream);
+        if (e == hipSuccess)
+          rc = msh_batch_extract(ctx, (int32_t)m, fs.data(), ps.data(), vp.data(), fp.data(),
+                                 MSH_DEVICE);
+        if (rc) break;
+        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) rc = hip_fail(c, e, "autotune");
+        if (rep == 1 || (rep > 1 && t < ms)) ms = t;
+      }
+      if (rc == MSH_OK && (best[g] < 0 || ms < best_ms)) {
+        best[g] = fam;
+        best_ms = ms;
+      }
+    }
+    c->family[method][T][cls] = best[g];
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  c->batch.have = false;  // the timing batches are not the caller's
+  c->have = false;
+  if (rc) return rc;
+  if (chosen)
+    for (int v = 0; v < n; ++v) chosen[v] = best[(size_t)group[v]];
   return MSH_OK;
 }
 

@@ -554,15 +554,21 @@ __global__ __launch_bounds__(kBlock) void k_emit_h5_b(const VolArgs* __restrict_
 // ---------------------------------------------------------------------------
 // Host side.
 
-// z chunk: 8 cells, and 4 for tetrahedra over Float64 (whose LDS tables and
-// brick would not fit twice per CU at 8); 16 measured slower for Float32
-// surface nets on dense fields (more chunks overflow the queue)
-template <int METHOD, typename T>
-static constexpr int h5_zc() { return (sizeof(T) == 8 && METHOD == MT) ? 4 : 8; }
+// z chunk: 8 cells for surface nets (16 measured slower on dense fields:
+// more chunks overflow the queue) and 4 for tetrahedra, whose LDS tables
+// leave room for two blocks per CU at 8 but three at 4 (Float32, 512^3 value
+// noise: 2.42 against 2.81 ms; 256^3 gyroid 0.178 against 0.182 ms).  The
+// half-depth bricks of surface nets (family 3: chunks of 4, three blocks per
+// CU become four) win on dense fields (512^3 value noise 1.29 against 1.43
+// ms) and lose on sparse ones (256^3 gyroid 0.147 against 0.131 ms), so the
+// autotuner chooses.
+template <int METHOD, typename T, bool HALF = false>
+static constexpr int h5_zc() { return METHOD == MT ? 4 : (HALF ? 4 : 8); }
 
 int h5_variant(const Plan& p, VolArgs* a) {
-  const int zc = p.method == SN ? (p.T == K_F32 ? h5_zc<SN, float>() : h5_zc<SN, double>())
-                                : (p.T == K_F32 ? h5_zc<MT, float>() : h5_zc<MT, double>());
+  const bool half = p.family == 3 && p.method == SN;
+  const int zc = p.method == SN ? (half ? h5_zc<SN, float, true>() : h5_zc<SN, float>())
+                                : h5_zc<MT, float>();
   const int64_t esz = p.T == K_F32 ? 4 : 8;
   // the brick's 32-bit DMA offsets reach plane zc+1, row kH5CY+1, sample 63
   if (((int64_t)(zc + 1) * a->g.nx * a->g.ny + (int64_t)(kH5CY + 1) * a->g.nx + 64) * esz >=
@@ -575,13 +581,13 @@ int h5_variant(const Plan& p, VolArgs* a) {
   // tile whose x is a multiple of 56, so that start is aligned too)
   const bool vec = ((uintptr_t)a->f % 16 == 0) && (((int64_t)a->g.nx * esz) % 16 == 0);
   const int cxclass = a->cxmask == 0 ? 1 : a->cxmask == 7 ? 2 : 0;
-  return (vec ? 0 : 1) | (cxclass << 1);
+  return (vec ? 0 : 1) | (cxclass << 1) | (half ? 8 : 0);
 }
 
-template <int METHOD, typename T, typename M, typename A, typename Out, int CXM>
+template <int METHOD, typename T, typename M, typename A, typename Out, int CXM, bool HALF = false>
 static hipError_t h5_z(bool vec, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
                        int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st) {
-  constexpr int ZC = h5_zc<METHOD, T>();
+  constexpr int ZC = h5_zc<METHOD, T, HALF>();
   if (vols) {
     const auto k = vec ? k_emit_h5_b<METHOD, T, M, A, Out, ZC, true, CXM>
                        : k_emit_h5_b<METHOD, T, M, A, Out, ZC, false, CXM>;
@@ -602,6 +608,16 @@ template <int METHOD, typename T, typename M, typename A, typename Out>
 static hipError_t h5_t(int variant, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
                        int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st) {
   const bool vec = (variant & 1) == 0;
+  if constexpr (METHOD == SN) {
+    if (variant & 8) {
+      switch ((variant >> 1) & 3) {
+        case 1: return h5_z<SN, T, M, A, Out, 0, true>(vec, a, vols, emap, gbase, nblocks, ctr, st);
+        case 2: return h5_z<SN, T, M, A, Out, 7, true>(vec, a, vols, emap, gbase, nblocks, ctr, st);
+        default:
+          return h5_z<SN, T, M, A, Out, -1, true>(vec, a, vols, emap, gbase, nblocks, ctr, st);
+      }
+    }
+  }
   switch ((variant >> 1) & 3) {
     case 1: return h5_z<METHOD, T, M, A, Out, 0>(vec, a, vols, emap, gbase, nblocks, ctr, st);
     case 2: return h5_z<METHOD, T, M, A, Out, 7>(vec, a, vols, emap, gbase, nblocks, ctr, st);

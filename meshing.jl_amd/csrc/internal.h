@@ -86,6 +86,16 @@ struct VolArgs {
   int64_t eblk0;           // first emit block of this volume in a batched launch
 };
 
+// Size classes of the kernel-family choice (msh_autotune keeps one choice
+// per method, element type and class): volumes of fewer than 2^20 cells
+// (about 100^3), fewer than 2^23 (about 200^3), and larger.
+constexpr int kSizeClasses = 3;
+inline int size_class(const msh_field* f) {
+  const double cells = (double)(f->nx > 1 ? f->nx - 1 : 0) * (double)(f->ny > 1 ? f->ny - 1 : 0) *
+                       (double)(f->nz > 1 ? f->nz - 1 : 0);
+  return cells < 1048576.0 ? 0 : cells < 8388608.0 ? 1 : 2;
+}
+
 constexpr int kScanItems = 8;
 constexpr int kScanTile = 256 * kScanItems;
 

@@ -828,7 +828,7 @@ int emit_variant(const Plan& p, VolArgs* a) {
 }
 
 int variant_family(int variant) {
-  if (variant >= 128 && variant < 192 && ((variant - 128) & 8)) return 3;  // half-depth bricks
+  if (variant >= 128 && ((variant & 63) & 8)) return 3;  // half-depth bricks (MC, SN)
   if (variant >= 128) return 2;
   if (variant >= 64 || variant < 32) return 1;
   return 0;

@@ -49,3 +49,22 @@ def test_autotune_keeps_results(pkg, oracle, reset_families, name, dtype):
     V, F = pkg.isosurface(dev_field(f), m)
     Vo, Fo = oracle.isosurface(f, name, iso=0)
     assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
+
+
+def test_autotune_batch_keeps_results(pkg, oracle, reset_families):
+    # the batch autotuner times each (method, element type, size class)
+    # group as a batched launch with every family; results do not change
+    rng = np.random.default_rng(21)
+    cases = []
+    for v in range(12):
+        shape = tuple(int(s) for s in rng.integers(12, 48, size=3))
+        dt = np.float64 if v % 4 == 0 else np.float32
+        cases.append((noise(shape, 40 + v, dt), ["mc", "mt", "sn"][v % 3]))
+    fields = [dev_field(f) for f, _ in cases]
+    methods = [method_of(pkg, name) for _, name in cases]
+    chosen = pkg.autotune_batch(fields, methods)
+    assert len(chosen) == len(cases) and all(c in (-1, 0, 1, 2, 3) for c in chosen)
+    res = pkg.isosurface_batch(fields, methods)
+    for (f, name), (V, F) in zip(cases, res):
+        Vo, Fo = oracle.isosurface(f, name, iso=0)
+        assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
