@@ -163,3 +163,24 @@ def test_repeated_batch_replays_graphs(pkg):
     assert torch.equal(res2[0][1], want[0][1])
     c2, e2 = _graph_launches(pkg)
     assert (c2, e2) == (c1, e1)
+
+
+def test_batch_workload_sizes_equal_single_calls(pkg):
+    # volumes drawn as in the batched workload (axes iid from 32..256,
+    # sphere / torus / gyroid / blobs, methods mixed, Float32), enough of
+    # them to span the size classes; each equals its single call
+    import torch
+    rng = np.random.default_rng(7)
+    kinds = ["sphere", "torus", "gyroid", "blobs"]
+    meths = [pkg.MarchingCubes(iso=0), pkg.MarchingTetrahedra(iso=0), pkg.NaiveSurfaceNets(iso=0)]
+    fields, methods = [], []
+    for v in range(40):
+        shape = tuple(int(x) for x in rng.integers(32, 257, size=3))
+        fields.append(pkg.generate_field(kinds[int(rng.integers(4))], shape, "float32", seed=v))
+        methods.append(meths[int(rng.integers(3))])
+    res = pkg.isosurface_batch(fields, methods)
+    for f, m, (V, F) in zip(fields, methods, res):
+        Vs, Fs = pkg.isosurface(f, m)
+        assert V.shape == Vs.shape and torch.equal(F, Fs)
+        ib = torch.int32 if V.dtype == torch.float32 else torch.int64
+        assert torch.equal(V.view(ib), Vs.view(ib))

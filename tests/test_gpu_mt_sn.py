@@ -80,3 +80,16 @@ def test_zsegments_and_big(pkg, oracle, name):
 @pytest.mark.parametrize("name", ["mt", "sn"])
 def test_host_arrays(pkg, oracle, name):
     run_both(pkg, oracle, noise((20, 30, 40), 1, np.float64), name, 0.0, host=True)
+
+
+@pytest.mark.parametrize("name", ["mt", "sn", "mc"])
+def test_gyroid_256_against_oracle(pkg, oracle, name):
+    # the tetrahedra / surface-nets workload at its real size: Float32 256^3
+    # gyroid on [-2pi, 2pi]^3, iso=0 (MarchingTetrahedra's default eps=1e-3)
+    m = {"mt": pkg.MarchingTetrahedra(iso=0), "sn": pkg.NaiveSurfaceNets(iso=0),
+         "mc": pkg.MarchingCubes(iso=0)}[name]
+    f = pkg.generate_field("gyroid", (256, 256, 256), "float32")
+    V, F = pkg.isosurface(f, m)
+    kw = dict(iso=0, eps=1e-3) if name == "mt" else dict(iso=0)
+    Vo, Fo = oracle.isosurface(f.cpu().numpy(), name, **kw)
+    assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
