@@ -47,12 +47,17 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
     uint16_t face[K ? 256 * 12 : 1];  // edges of each face, 5 bits each
     uint8_t rank0[K ? 256 * 19 : 1];  // rank of each edge among a class-0 voxel's vertices
     uint8_t own0[K ? 256 * 8 : 1];    // edges of a class-0 voxel's vertices, in order
-    uint8_t shift[K ? 152 : 1];       // MT_SHIFT
-    uint8_t cc[NS][K ? ZC : 1][64];   // case of each cell of the chunk
+    // owner step of a face vertex, by edge and which of the cells at -x,
+    // -y, -z exist (bits 0-2): the edge's index in its owner | the step
+    // (dx, dy, dz) << 5
+    uint8_t own_step[K ? 152 : 1];
+    // in-row vertex prefix << 8 | case of each cell of the chunk (one LDS
+    // word per owner look-up)
+    uint32_t cpc[NS][K ? ZC : 1][64];
     uint8_t pc[NS][K ? 64 : 1];       // case of the cell just below the chunk
     uint32_t edge[19];                // per edge: corner offsets and axis bits
     uint16_t xmask[K ? 1 : 256];      // surface nets: crossing cube edges, by natural case
-    uint16_t cp[NS][ZC][64];          // in-row vertex prefix of each cell of the chunk
+    uint16_t cp[NS][K ? 1 : ZC][64];  // surface nets: in-row vertex prefix of each cell
     int64_t rb[NS][64];               // row vertex offset at the chunk start
     int64_t dv[kWaves][64], df[kWaves][64];  // per row: output index minus queue slot
     // queue entries: x | z << 6 | case << 10 | class << 18 | first slot << 21
@@ -79,7 +84,15 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
       for (int e = 0; e < 19; ++e) sm.rank0[c * 19 + e] = MT_RANK[m * 19 + e];
       for (int q = 0; q < 8; ++q) sm.own0[c * 8 + q] = q < 7 ? MT_OWN_EDGES[m * 20 + q] : 0;
     }
-    for (int q = threadIdx.x; q < 152; q += kBlock) sm.shift[q] = MT_SHIFT[q];
+    for (int q = threadIdx.x; q < 152; q += kBlock) {
+      // the owner is one cell back along each axis on which the edge's end
+      // corner has coordinate 0, when that cell exists (bits of q & 7)
+      const int ed = q >> 3;
+      const int cb = MT_EDGE[ed] >> 4;
+      const uint32_t d = ((mt_dx(cb) == 0 ? 1u : 0u) | (mt_dy(cb) == 0 ? 2u : 0u) |
+                          (mt_dz(cb) == 0 ? 4u : 0u)) & (uint32_t)(q & 7);
+      sm.own_step[q] = (uint8_t)(MT_SHIFT[ed * 8 + d] | (d << 5));
+    }
   }
   if (threadIdx.x < (METHOD == MT ? 19 : 12)) {
     // edge e from corner ca to cb (tetrahedron edges in MT corner order, or
@@ -237,7 +250,7 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
         if constexpr (METHOD == MT) {
           nvz[z] = sm.nown[cls * 256 + cs[z]];
           nfz[z] = emits ? (uint32_t)sm.nf[cs[z]] : 0u;
-          sm.cc[sl][z][lane] = (uint8_t)cs[z];
+          sm.cpc[sl][z][lane] = (runv << 8) | cs[z];
         } else {
           const uint32_t csm = nat_to_mc(cs[z]), c0 = csm & 1;
           nvz[z] = (cs[z] != 0 && cs[z] != 255) ? 1u : 0u;
@@ -245,8 +258,8 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
                               (uint32_t)((c0 != ((csm >> 3) & 1)) && gk >= 1 && gi >= 1) +
                               (uint32_t)((c0 != ((csm >> 4) & 1)) && gi >= 1 && gj >= 1);
           nfz[z] = (emits && nvz[z]) ? 2u * nq : 0u;
+          sm.cp[sl][z][lane] = (uint16_t)runv;
         }
-        sm.cp[sl][z][lane] = (uint16_t)runv;
         runv += nvz[z];
         if (emits) tot += nvz[z] | (nfz[z] << 16);
       }
@@ -271,11 +284,11 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
           if constexpr (METHOD == MT) {
             const uint32_t cls = (uint32_t)((gi == 0) | ((y0 == 0) << 1) | ((zc + z == 0) << 2));
             nv = sm.nown[cls * 256 + c];
-            sm.cc[0][z][lane] = (uint8_t)c;
+            sm.cpc[0][z][lane] = (hrun << 8) | c;
           } else {
             nv = (c != 0 && c != 255) ? 1u : 0u;
+            sm.cp[0][z][lane] = (uint16_t)hrun;
           }
-          sm.cp[0][z][lane] = (uint16_t)hrun;
           hrun += nv;
         }
         if constexpr (METHOD == MT) sm.pc[0][lane] = (uint8_t)((hexists && zc > 0) ? (hn[0] | (hn[1] << 4)) : 0u);
@@ -374,24 +387,24 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
             int64_t id[3];
             if constexpr (METHOD == MT) {
               const uint32_t fw = sm.face[ecs * 12 + q];
+              const uint32_t nb = (uint32_t)((ci > 0) | ((cj > 0) << 1) | ((ck > 0) << 2));
   #pragma unroll
               for (int m = 0; m < 3; ++m) {
                 const uint32_t ed = (fw >> (5 * m)) & 31;
-                const uint32_t ew = sm.edge[ed];
-                // the owner is one cell back along each axis on which the edge's
-                // end corner has coordinate 0 (when that cell exists)
-                const int dx = (((ew >> 23) & 1) == 0) & (ci > 0);
-                const int dy = (((ew >> 24) & 1) == 0) & (cj > 0);
-                const int dz = (((ew >> 25) & 1) == 0) & (ck > 0);
-                const uint32_t e2 = sm.shift[ed * 8 + (dx | (dy << 1) | (dz << 2))];
+                // the owner (src/marching_tetrahedra.jl:67-84: the first voxel
+                // to request the edge) and the edge's index there
+                const uint32_t st = sm.own_step[ed * 8 + nb];
+                const uint32_t e2 = st & 31;
+                const int dx = (st >> 5) & 1, dy = (st >> 6) & 1, dz = (st >> 7) & 1;
                 const int l2 = lc - dx, w2 = sl - dy, z2 = c - dz;
                 const uint32_t cls2 =
                     (uint32_t)((ci - dx == 0) | ((cj - dy == 0) << 1) | ((ck - dz == 0) << 2));
                 uint32_t oc;
                 int64_t off;
                 if (z2 >= 0) {
-                  oc = sm.cc[w2][z2][l2];
-                  off = sm.rb[w2][l2] + sm.cp[w2][z2][l2];
+                  const uint32_t w = sm.cpc[w2][z2][l2];
+                  oc = w & 0xFF;
+                  off = sm.rb[w2][l2] + (int64_t)(w >> 8);
                 } else {
                   oc = sm.pc[w2][l2];
                   off = sm.rb[w2][l2] - (int64_t)sm.nown[cls2 * 256 + oc];
