@@ -487,8 +487,10 @@ def traffic_of(workload, family_id):
 # family (family 3 halves it)
 _EMIT_NAMES = {"mc": {3: "k_emit_mc5", 2: "k_emit_mc5", 1: "k_emit_mc4", 0: "k_emit"},
                "mt": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"},
-               "sn": {2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"}}
+               "sn": {3: "k_emit_h5", 2: "k_emit_h5", 1: "k_emit_halo", 0: "k_emit"}}
 _MC5_ZC = {("float32", 2): 16, ("float32", 3): 8, ("float64", 2): 8, ("float64", 3): 4}
+# brick depth of the tetrahedra / surface-nets emit per family
+_H5_ZC = {("mt", 2): 4, ("sn", 2): 8, ("sn", 3): 4}
 
 
 def _emit_pattern(method, dtype, family_id):
@@ -502,6 +504,9 @@ def _emit_pattern(method, dtype, family_id):
     mid = {"mc": 0, "mt": 1, "sn": 2}[method]
     if name == "k_emit":
         return re.compile(rf"k_emit<{mid}, {t},")
+    if name == "k_emit_h5":
+        return re.compile(rf"k_emit_h5<{mid}, {t}, [a-z]+, [a-z]+, [a-z]+, "
+                          rf"{_H5_ZC[(method, family_id)]},")
     return re.compile(rf"{name}<{mid}, {t},")
 
 

@@ -22,17 +22,22 @@ def test_summary_selects_timed_instantiation(tmp_path):
     f32_16 = "msh::(anonymous namespace)::k_emit_mc5<float, float, float, 16, true, 0>(msh::VolArgs)"
     f32_8 = "msh::(anonymous namespace)::k_emit_mc5<float, float, float, 8, true, 0>(msh::VolArgs)"
     f64_8 = "msh::(anonymous namespace)::k_emit_mc5<double, double, double, 8, true, 0>(msh::VolArgs)"
-    h5mt = "msh::(anonymous namespace)::k_emit_h5<1, float, float, double, double, 8, true, 0>(msh::VolArgs)"
+    h5mt = "msh::(anonymous namespace)::k_emit_h5<1, float, float, double, double, 4, true, 0>(msh::VolArgs)"
+    h5sn8 = "msh::(anonymous namespace)::k_emit_h5<2, float, float, float, float, 8, true, 0>(msh::VolArgs)"
+    h5sn4 = "msh::(anonymous namespace)::k_emit_h5<2, float, float, float, float, 4, true, 0>(msh::VolArgs)"
     copy = "__amd_rocclr_copyBuffer"
     fetch = [(f64_8, 1000, 100.0), (f32_16, 900, 40.0), (f32_16, 900, 42.0), (f32_8, 900, 99.0),
-             (f32_16, 20, 1.0), (h5mt, 50, 7.0)] + [(copy, 1, 524288.0)] * 4
+             (f32_16, 20, 1.0), (h5mt, 50, 7.0), (h5sn4, 50, 3.0), (h5sn8, 50, 9.0)] + \
+        [(copy, 1, 524288.0)] * 4
     write = [(f64_8, 1000, 80.0), (f32_16, 900, 30.0), (f32_16, 900, 30.0), (f32_8, 900, 77.0),
-             (f32_16, 20, 2.0), (h5mt, 50, 5.0)] + [(copy, 1, 1048576.0)] * 4
+             (f32_16, 20, 2.0), (h5mt, 50, 5.0), (h5sn4, 50, 1.0), (h5sn8, 50, 4.0)] + \
+        [(copy, 1, 1048576.0)] * 4
     _csv(tmp_path / "f", "FETCH_SIZE", fetch)
     _csv(tmp_path / "w", "WRITE_SIZE", write)
     line = {"float64": {"family_id": 2, "algorithmic_bytes": 1000},
             "float32": {"family_id": 2, "algorithmic_bytes": 1000},
             "gyroid_256_f32": {"mt": {"family_id": 2, "roofline": {"algorithmic_bytes_per_launch": 10}},
+                               "sn": {"family_id": 2, "roofline": {"algorithmic_bytes_per_launch": 10}},
                                "mc": {"family_id": 2, "roofline": {"algorithmic_bytes_per_launch": 10}}}}
     (tmp_path / "line.json").write_text("noise\n" + json.dumps(line) + "\n")
     out = bench.summarize_counters(tmp_path / "f", tmp_path / "w", tmp_path / "line.json",
@@ -45,6 +50,7 @@ def test_summary_selects_timed_instantiation(tmp_path):
     assert w["mc_value_noise_1024_float64"]["kernel"] == f64_8
     assert w["mc_gyroid_256_float32"]["grid"] == 20  # the small launch of the same kernel
     assert w["mt_gyroid_256_float32"]["kernel"] == h5mt
+    assert w["sn_gyroid_256_float32"]["kernel"] == h5sn8  # family 2: full-depth bricks
     # a summary from another family is not used for the timed kernel
     bench_t = json.loads((tmp_path / "t.json").read_text())
     assert bench_t["workloads"]["mc_value_noise_1024_float32"]["family_id"] == 2
