@@ -27,8 +27,9 @@ constexpr int kH5TX = 56;  // cells a block emits along x: lanes 4..59
 constexpr int kH5CY = 4;   // cell rows a block emits along y: one per wave
 
 template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM,
-          class Src>
+          int NBUF, class Src>
 __device__ __forceinline__ void emit_h5_run(Src src) {
+  static_assert(NBUF == 1 || NBUF == 2, "single- or double-buffered bricks");
   static_assert(METHOD == MT || METHOD == SN, "tetrahedra or surface nets");
   static_assert(ZC <= 16, "z fits 4 bits of a queue entry");
   constexpr int SX = 64, SY = kH5CY + 2, NP = ZC + 2;  // planes zc-1 .. zc+ZC, rows jb-1 .. jb+4
@@ -38,8 +39,8 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
   static_assert(kH5TX * 13 <= CAPV && kH5TX * 12 <= CAPF && kH5TX <= CAPE, "one z step must fit");
   constexpr int K = METHOD == MT ? 1 : 0;
   struct Smem {
-    uint32_t smp[2][NP * SY * RD];
-    double cz[2][NP];
+    uint32_t smp[NBUF][NP * SY * RD];
+    double cz[NBUF][NP];
     double cx[SX], cy[SY];
     // tetrahedra only (size 1 for surface nets)
     uint8_t nown[K ? 8 * 256 : 1];    // vertices created, by class and (natural) case
@@ -225,8 +226,10 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
     __builtin_amdgcn_s_waitcnt(0);
     lds_barrier();
     int buf = 0;
-    for (int zc = zs; zc < ze; zc += ZC, buf ^= 1) {
-      if (zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
+    for (int zc = zs; zc < ze; zc += ZC, buf ^= NBUF - 1) {
+      // double-buffered: the next brick streams in during this chunk;
+      // single-buffered: it is fetched after the chunk (more blocks per CU)
+      if (NBUF == 2 && zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
       const uint32_t* smp = sm.smp[buf];
       const double* czs = sm.cz[buf];
 
@@ -534,6 +537,11 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
       rbv += runv;
       rbf += tot >> 16;
       lds_barrier();
+      if (NBUF == 1 && zc + ZC < ze) {
+        fetch(zc + ZC, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        lds_barrier();
+      }
     }
   };
   if constexpr (Src::kOne) {
@@ -545,20 +553,22 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
   }
 }
 
-template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM>
+template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM,
+          int NBUF>
 __global__ __launch_bounds__(kBlock) void k_emit_h5(VolArgs a) {
-  emit_h5_run<METHOD, T, M, A, Out, ZC, VEC, CXM>(
+  emit_h5_run<METHOD, T, M, A, Out, ZC, VEC, CXM, NBUF>(
       OneTile{&a, xcd_tile(blockIdx.x, a.ntiles), false});
 }
 
 // Batched, persistent: the launch's blocks take the tiles of its volumes
 // from a work queue (TileQueue).
-template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM>
+template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM,
+          int NBUF>
 __global__ __launch_bounds__(kBlock) void k_emit_h5_b(const VolArgs* __restrict__ vols,
                                                       const int32_t* __restrict__ emap,
                                                       int64_t gbase, int64_t ntiles,
                                                       uint32_t* __restrict__ ctr) {
-  emit_h5_run<METHOD, T, M, A, Out, ZC, VEC, CXM>(
+  emit_h5_run<METHOD, T, M, A, Out, ZC, VEC, CXM, NBUF>(
       TileQueue<true>{vols, emap, gbase, ntiles, ctr, 0});
 }
 
@@ -577,6 +587,13 @@ __global__ __launch_bounds__(kBlock) void k_emit_h5_b(const VolArgs* __restrict_
 // autotuner chooses.
 template <int METHOD, typename T, bool HALF = false>
 static constexpr int h5_zc() { return METHOD == MT ? 4 : (HALF ? 4 : 8); }
+// bricks per block: 2 for tetrahedra (the next chunk's brick streams in
+// during this one), 1 for surface nets, whose lighter LDS then holds four
+// blocks per CU instead of three (256^3 gyroid 0.115 against 0.130 ms,
+// 512^3 value noise 1.26 against 1.30 ms; tetrahedra single-buffered were
+// slower, 2.49 against 2.36 ms)
+template <int METHOD, typename T, bool HALF = false>
+static constexpr int h5_nbuf() { return METHOD == SN ? 1 : 2; }
 
 int h5_variant(const Plan& p, VolArgs* a) {
   const bool half = p.family == 3 && p.method == SN;
@@ -601,18 +618,19 @@ template <int METHOD, typename T, typename M, typename A, typename Out, int CXM,
 static hipError_t h5_z(bool vec, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
                        int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st) {
   constexpr int ZC = h5_zc<METHOD, T, HALF>();
+  constexpr int NB = h5_nbuf<METHOD, T, HALF>();
   if (vols) {
-    const auto k = vec ? k_emit_h5_b<METHOD, T, M, A, Out, ZC, true, CXM>
-                       : k_emit_h5_b<METHOD, T, M, A, Out, ZC, false, CXM>;
+    const auto k = vec ? k_emit_h5_b<METHOD, T, M, A, Out, ZC, true, CXM, NB>
+                       : k_emit_h5_b<METHOD, T, M, A, Out, ZC, false, CXM, NB>;
     hipLaunchKernelGGL(k, dim3(persistent_grid((const void*)k, nblocks)), dim3(kBlock), 0, st,
                        vols, emap, gbase, nblocks, ctr);
   } else {
     if (vec)
-      hipLaunchKernelGGL((k_emit_h5<METHOD, T, M, A, Out, ZC, true, CXM>), dim3((unsigned)a.ntiles),
-                         dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((k_emit_h5<METHOD, T, M, A, Out, ZC, true, CXM, NB>),
+                         dim3((unsigned)a.ntiles), dim3(kBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((k_emit_h5<METHOD, T, M, A, Out, ZC, false, CXM>), dim3((unsigned)a.ntiles),
-                         dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((k_emit_h5<METHOD, T, M, A, Out, ZC, false, CXM, NB>),
+                         dim3((unsigned)a.ntiles), dim3(kBlock), 0, st, a);
   }
   return hipGetLastError();
 }

@@ -30,8 +30,9 @@ namespace {
 constexpr int kMc5TX = 60;  // cells along x per block (lanes 60..63 only load)
 constexpr int kMc5CY = 4;   // cell rows along y per block: one per wave
 
-template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM, class Src>
+template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM, int NBUF, class Src>
 __device__ __forceinline__ void emit_mc5_run(Src src) {
+  static_assert(NBUF == 1 || NBUF == 2, "single- or double-buffered bricks");
   constexpr int TX = kMc5TX, SX = 64, SY = kMc5CY + 1;
   constexpr int NP = ZC + 1;
   constexpr int DW = (int)(sizeof(T) / 4);
@@ -39,8 +40,8 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
   constexpr int CAPE = 256, CAPV = 1024, CAPF = 512;  // per wave: entries, vertex and face slots
   static_assert(TX * 12 <= CAPV && TX * 5 <= CAPF && TX <= CAPE, "one z step must fit");
   struct Smem {
-    uint32_t smp[2][NP * SY * RD];
-    double cz[2][NP];
+    uint32_t smp[NBUF][NP * SY * RD];
+    double cz[NBUF][NP];
     double cx[SX], cy[SY];
     uint32_t cnt[256];
     uint64_t lv[256], lt[256];
@@ -165,8 +166,10 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
     __builtin_amdgcn_s_waitcnt(0);
     lds_barrier();
     int buf = 0;
-    for (int zc = zs; zc < ze; zc += ZC, buf ^= 1) {
-      if (zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
+    for (int zc = zs; zc < ze; zc += ZC, buf ^= NBUF - 1) {
+      // double-buffered: the next brick streams in during this chunk;
+      // single-buffered: it is fetched after the chunk (more blocks per CU)
+      if (NBUF == 2 && zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
       const uint32_t* smp = sm.smp[buf];
       const double* czs = sm.cz[buf];
 
@@ -318,6 +321,11 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
       rbv += tot & 0xFFFF;
       rbf += tot >> 16;
       lds_barrier();
+      if (NBUF == 1 && zc + ZC < ze) {
+        fetch(zc + ZC, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        lds_barrier();
+      }
     }
   };
   if constexpr (Src::kOne) {
@@ -329,19 +337,21 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
   }
 }
 
-template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM>
+template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM, int NBUF>
 __global__ __launch_bounds__(kBlock) void k_emit_mc5(VolArgs a) {
-  emit_mc5_run<T, M, Out, ZC, VEC, CXM>(OneTile{&a, xcd_tile(blockIdx.x, a.ntiles), false});
+  emit_mc5_run<T, M, Out, ZC, VEC, CXM, NBUF>(
+      OneTile{&a, xcd_tile(blockIdx.x, a.ntiles), false});
 }
 
 // Batched, persistent: the launch's blocks take the tiles of its volumes
 // from a work queue (TileQueue).
-template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM>
+template <typename T, typename M, typename Out, int ZC, bool VEC, int CXM, int NBUF>
 __global__ __launch_bounds__(kBlock) void k_emit_mc5_b(const VolArgs* __restrict__ vols,
                                                        const int32_t* __restrict__ emap,
                                                        int64_t gbase, int64_t ntiles,
                                                        uint32_t* __restrict__ ctr) {
-  emit_mc5_run<T, M, Out, ZC, VEC, CXM>(TileQueue<true>{vols, emap, gbase, ntiles, ctr, 0});
+  emit_mc5_run<T, M, Out, ZC, VEC, CXM, NBUF>(
+      TileQueue<true>{vols, emap, gbase, ntiles, ctr, 0});
 }
 
 }  // namespace
@@ -351,6 +361,13 @@ __global__ __launch_bounds__(kBlock) void k_emit_mc5_b(const VolArgs* __restrict
 
 template <typename T>
 static constexpr int mc5_zc() { return sizeof(T) == 4 ? 16 : 8; }
+// bricks per block: 2 for the full-depth chunks (the next chunk's brick
+// streams in during this one), 1 for the half-depth ones of family 3, which
+// then fit five blocks per CU: on the sparse 256^3 gyroid 0.056 ms against
+// 0.062 double-buffered (and 0.073 full-depth), while the dense 1024^3 field
+// keeps the double-buffered full depth (4.67 ms; single-buffered 4.96)
+template <typename T, int ZC>
+static constexpr int mc5_nbuf() { return ZC == mc5_zc<T>() / 2 ? 1 : 2; }
 
 // Half-depth bricks (family 3): the z chunk of the all-Float32 and
 // all-Float64 instantiations with Float64 axes is halved (a smaller LDS
@@ -372,17 +389,18 @@ int mc5_variant(const Plan& p, VolArgs* a) {
 template <typename T, typename M, typename Out, int CXM, int ZC = mc5_zc<T>()>
 static hipError_t mc5_z(bool vec, const VolArgs& a, const VolArgs* vols, const int32_t* emap,
                         int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st) {
+  constexpr int NB = mc5_nbuf<T, ZC>();
   if (vols) {
-    const auto k =
-        vec ? k_emit_mc5_b<T, M, Out, ZC, true, CXM> : k_emit_mc5_b<T, M, Out, ZC, false, CXM>;
+    const auto k = vec ? k_emit_mc5_b<T, M, Out, ZC, true, CXM, NB>
+                       : k_emit_mc5_b<T, M, Out, ZC, false, CXM, NB>;
     hipLaunchKernelGGL(k, dim3(persistent_grid((const void*)k, nblocks)), dim3(kBlock), 0, st,
                        vols, emap, gbase, nblocks, ctr);
   } else {
     if (vec)
-      hipLaunchKernelGGL((k_emit_mc5<T, M, Out, ZC, true, CXM>), dim3((unsigned)a.ntiles),
+      hipLaunchKernelGGL((k_emit_mc5<T, M, Out, ZC, true, CXM, NB>), dim3((unsigned)a.ntiles),
                          dim3(kBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((k_emit_mc5<T, M, Out, ZC, false, CXM>), dim3((unsigned)a.ntiles),
+      hipLaunchKernelGGL((k_emit_mc5<T, M, Out, ZC, false, CXM, NB>), dim3((unsigned)a.ntiles),
                          dim3(kBlock), 0, st, a);
   }
   return hipGetLastError();
