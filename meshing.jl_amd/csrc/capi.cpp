@@ -1099,15 +1099,15 @@ int msh_weld(void* ctx, int32_t kind, int64_t nverts, const void* verts, int64_t
   if (nverts > ((int64_t)3 << 30) || nfaces > (int64_t)1 << 40)
     return fail(c, MSH_ERR_ARGUMENT, "mesh too large (at most 3 * 2^30 vertices)");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSH_ERR_DEVICE, "hipSetDevice");
-  // workspace: flags | prefixes | first appearances | tiles | sizes | hash table
+  // workspace: flags | prefixes | first appearances | tiles | sizes | region bounds | hash table
   int64_t nslots = 1;
   while (nslots < nverts + nverts / 4 + 1) nslots <<= 1;
   const int64_t U = nverts > 0 ? nverts : 1;
   const int64_t T = (U + 4095) / 4096;
   const size_t b_flag = align256((size_t)U * 4), b_i64 = align256((size_t)U * 8);
   const size_t b_tcnt = align256((size_t)(T + 1) * 4), b_toff = align256((size_t)T * 8);
-  const size_t b_tot = 256, b_table = align256((size_t)nslots * 8);
-  const size_t need = b_flag + 2 * b_i64 + b_tcnt + b_toff + b_tot + b_table;
+  const size_t b_tot = 256, b_bnd = 512, b_table = align256((size_t)nslots * 8);
+  const size_t need = b_flag + 2 * b_i64 + b_tcnt + b_toff + b_tot + b_bnd + b_table;
   int rc = grow(c, &c->wws, &c->wws_bytes, need, "weld workspace allocation");
   if (rc) return rc;
   WeldArgs a{};
@@ -1126,6 +1126,8 @@ int msh_weld(void* ctx, int32_t kind, int64_t nverts, const void* verts, int64_t
   a.total = sizes;
   a.bad = sizes + 1;
   q += b_tot;
+  a.bounds = (uint64_t*)q;
+  q += b_bnd;
   a.table = (int64_t*)q;
   a.out_kind = kind;
   a.nverts = nverts;

@@ -154,6 +154,7 @@ struct WeldArgs {
   int64_t* faces;        // nfaces x 3, 1-based, renumbered in place
   int64_t* table;        // hash slots: a power of two >= 1.25 nverts
   int64_t nslots;
+  uint64_t* bounds;      // room for 63 x-region bounds of a large table
   int64_t* rep;          // nverts: index of the vertex's first appearance
   uint32_t* flag;        // nverts: 1 at first appearances
   int64_t* pos;          // nverts: exclusive prefix of flag

@@ -7,6 +7,9 @@
 // the faces are renumbered (1-based) to them, so the result is what a
 // first-appearance `unique` over the vertex tuples gives on the host.
 //
+//   k_weld_bounds   (large tables only) one block sorts 4096 x coordinates
+//                   sampled evenly over the vertices and keeps their
+//                   quantiles as the bounds of the table's 64 x regions
 //   k_weld_insert   open-addressing hash table over the coordinate bits,
 //                   first within each block's 512 vertices in LDS, then
 //                   over the whole mesh for the block's first appearances:
@@ -26,6 +29,18 @@
 //   k_weld_faces    face indices renumbered in place to the new index of
 //                   their vertex's first appearance (out-of-range indices
 //                   are counted and left as they are)
+//
+// A key's home slot in a large table is its x region's block of slots plus
+// the hash's low bits inside it.  Vertices of a mesh are emitted cell by
+// cell with x outermost, so the blocks in flight at any moment work in one
+// or two of the 64 regions rather than all over the table (on the 272 M
+// vertex Marching Cubes mesh of 1024^3 value noise: Float64 7% faster,
+// Float32 level; 512 regions and none were slower); the regions come from
+// quantiles of the x values, so each takes a like share of the keys whatever
+// their distribution, and keys whose x equals several bounds (a plane
+// x = const) spread over those regions by the hash.  Placement only changes
+// where keys sit: the table still ends holding each key's smallest index,
+// and a probe that runs past its region goes on into the next one.
 //
 // Every probe loop ends: each table has more slots than keys (the global
 // one at least 1.25 times the vertices, the local one twice its window),
@@ -59,12 +74,79 @@ __global__ void k_fill_i64(int64_t* __restrict__ p, int64_t n, int64_t v) {
   if (i < n) p[i] = v;
 }
 
-// Global insert of vertex i with key (x, y, z): returns its slot.
+// An IEEE bit pattern as an unsigned integer whose order is the numeric
+// order (-0 before +0, NaNs beyond the infinities of their sign).
+__device__ __forceinline__ uint64_t ordered(uint32_t b) {
+  return (b & 0x80000000u) ? (uint64_t)(uint32_t)~b : (uint64_t)(b | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t ordered(uint64_t b) {
+  return (b >> 63) ? ~b : b | (1ull << 63);
+}
+
+constexpr int kWeldSample = 4096;     // x coordinates sorted for the region bounds
+constexpr int kWeldRegions = 64;      // regions of a large table (measured: 64 beat 512 and none)
+constexpr int kWeldSortBlock = 1024;
+
+// One block: bounds[r - 1] = the (r / nreg) quantile of the sampled x
+// coordinates (as ordered integers), r = 1 .. nreg - 1 (bitonic sort in LDS).
+template <typename W>
+__global__ __launch_bounds__(kWeldSortBlock) void k_weld_bounds(const W* __restrict__ v,
+                                                               int64_t nv, int nreg,
+                                                               uint64_t* __restrict__ bounds) {
+  __shared__ uint64_t s[kWeldSample];
+  for (int k = threadIdx.x; k < kWeldSample; k += kWeldSortBlock)
+    s[k] = ordered(v[3 * ((int64_t)k * nv / kWeldSample)]);
+  __syncthreads();
+  for (int size = 2; size <= kWeldSample; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = threadIdx.x; k < kWeldSample; k += kWeldSortBlock) {
+        const int p = k ^ stride;
+        if (p > k) {
+          const uint64_t a = s[k], b = s[p];
+          if ((a > b) == ((k & size) == 0)) {  // ascending runs where k & size is 0
+            s[k] = b;
+            s[p] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int r = threadIdx.x + 1; r < nreg; r += kWeldSortBlock)
+    bounds[r - 1] = s[(int64_t)r * kWeldSample / nreg];
+}
+
+// Home slot of a key with hash h: region (from x and the nreg - 1 sorted
+// bounds in bnd) in the bits from rbits up, the hash's low bits below.
+template <typename W>
+__device__ __forceinline__ uint64_t home_slot(W x, uint64_t h, const uint64_t* bnd, int nreg,
+                                              int rbits) {
+  const uint64_t low = h & ((1ull << rbits) - 1);
+  if (nreg == 1) return low;
+  const uint64_t o = ordered(x);
+  int a = 0, b = nreg - 1;  // c = bounds below o
+  while (a < b) {
+    const int m = (a + b) >> 1;
+    if (bnd[m] < o) a = m + 1; else b = m;
+  }
+  const int c = a;
+  uint64_t r = c;
+  if (c < nreg - 1 && bnd[c] == o) {  // o is a bound: e = bounds at or below o
+    b = nreg - 1;
+    while (a < b) {
+      const int m = (a + b) >> 1;
+      if (bnd[m] <= o) a = m + 1; else b = m;
+    }
+    r += (h >> 32) % (uint64_t)(a - c + 1);
+  }
+  return (r << rbits) | low;
+}
+
+// Global insert of vertex i with key (x, y, z), probing from slot h:
+// returns its slot.
 template <typename W>
 __device__ __forceinline__ uint32_t global_insert(const W* __restrict__ v, int64_t i, W x, W y,
-                                                  W z, unsigned long long* table,
+                                                  W z, uint64_t h, unsigned long long* table,
                                                   uint64_t mask) {
-  uint64_t h = key_hash(x, y, z) & mask;
   while (true) {
     // a plain load first: a CAS on a taken slot measured slower
     unsigned long long cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED,
@@ -98,15 +180,19 @@ template <typename W>
 __global__ __launch_bounds__(kBlock) void k_weld_insert(const W* __restrict__ v, int64_t nv,
                                                         unsigned long long* table,
                                                         uint64_t mask,
+                                                        const uint64_t* __restrict__ bounds,
+                                                        int nreg, int rbits,
                                                         uint32_t* __restrict__ slot,
                                                         uint16_t* __restrict__ lrep) {
   __shared__ W key[3 * kWeldLT];
   __shared__ uint32_t lt[kWeldLS];  // local slot -> smallest local index of its key
+  __shared__ uint64_t bnd[kWeldRegions - 1];
   constexpr uint32_t kEmpty = 0xFFFFFFFFu;
   constexpr int PER = kWeldLT / kBlock;
   const int64_t base = (int64_t)blockIdx.x * kWeldLT;
   const int n = nv - base < kWeldLT ? (int)(nv - base) : kWeldLT;
   for (int k = threadIdx.x; k < kWeldLS; k += kBlock) lt[k] = kEmpty;
+  for (int k = threadIdx.x; k < nreg - 1; k += kBlock) bnd[k] = bounds[k];
   for (int k = threadIdx.x; k < 3 * n; k += kBlock) key[k] = v[3 * base + k];
   __syncthreads();
   uint32_t ls[PER];
@@ -138,9 +224,11 @@ __global__ __launch_bounds__(kBlock) void k_weld_insert(const W* __restrict__ v,
     if (j >= n) continue;
     const uint32_t l = lt[ls[q]];
     lrep[base + j] = (uint16_t)l;
-    if (l == (uint32_t)j)
-      slot[base + j] = global_insert(v, base + j, key[3 * j], key[3 * j + 1], key[3 * j + 2],
-                                     table, mask);
+    if (l == (uint32_t)j) {
+      const W x = key[3 * j], y = key[3 * j + 1], z = key[3 * j + 2];
+      const uint64_t h = home_slot(x, key_hash(x, y, z), bnd, nreg, rbits);
+      slot[base + j] = global_insert(v, base + j, x, y, z, h, table, mask);
+    }
   }
 }
 
@@ -280,8 +368,19 @@ hipError_t weld_t(const WeldArgs& a, hipStream_t st) {
     uint32_t* slot = (uint32_t*)a.pos;
     uint16_t* lrep = (uint16_t*)(slot + a.nverts);
     const unsigned nwin = (unsigned)((a.nverts + kWeldLT - 1) / kWeldLT);
+    // x regions only where the table outgrows the caches (2^23 slots,
+    // 64 MiB, and up)
+    int lg = 0;
+    while (((int64_t)1 << lg) < a.nslots) ++lg;
+    const int nreg = lg >= 23 ? kWeldRegions : 1;
+    int rbits = lg;
+    for (int r = nreg; r > 1; r >>= 1) --rbits;
+    if (nreg > 1)
+      hipLaunchKernelGGL((k_weld_bounds<W>), dim3(1), dim3(kWeldSortBlock), 0, st, v, a.nverts,
+                         nreg, a.bounds);
     hipLaunchKernelGGL((k_weld_insert<W>), dim3(nwin), dim3(kBlock), 0, st, v, a.nverts,
-                       (unsigned long long*)a.table, (uint64_t)(a.nslots - 1), slot, lrep);
+                       (unsigned long long*)a.table, (uint64_t)(a.nslots - 1),
+                       (const uint64_t*)a.bounds, nreg, rbits, slot, lrep);
     const int64_t ntiles = (a.nverts + kWeldTile - 1) / kWeldTile;
     hipLaunchKernelGGL(k_fill_i64, dim3(grid_of((ntiles + 1) / 2)), dim3(kBlock), 0, st,
                        (int64_t*)a.tilecnt, (ntiles + 1) / 2, (int64_t)0);

@@ -88,6 +88,32 @@ def test_weld_value_noise_256(pkg, oracle, dtype):
     assert torch.equal(Vw[Fw - 1].view(torch.uint8), V[F - 1].view(torch.uint8))
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_weld_large_table_skewed_x(pkg, oracle, dtype):
+    # from 2^23 hash slots up, keys are placed by x region (quantiles of a
+    # sample of the x coordinates): skewed x, a plane x = const holding half
+    # the points, signed zeros, NaNs and an order without any locality
+    # still weld to the definition
+    rng = np.random.default_rng(7)
+    n = 3_500_000  # 1.25 n rounds up to 2^23 slots
+    pts = rng.standard_normal((1_200_000, 3)).astype(dtype)
+    pts[:600_000, 0] = 0.25
+    pts[600_000:700_000, 0] = -0.0
+    pts[700_000:800_000, 0] = 0.0
+    pts[800_000:810_000, 0] = np.nan
+    pts[810_000:820_000, 0] = -np.inf
+    V = pts[rng.integers(0, len(pts), n)]
+    F = rng.integers(1, n + 1, (1_000_000, 3))
+    weld_both(pkg, oracle, V, F)
+    # every x the same: every key spreads over all regions by its hash
+    V[:, 0] = 1.0
+    weld_both(pkg, oracle, V, F)
+    # x sorted (the order of an extraction) and 2^24 slots
+    V2 = np.concatenate([pts[rng.integers(0, len(pts), 4_000_000)], V[:3_000_000]])
+    V2 = V2[np.argsort(V2[:, 0], kind="stable")]
+    weld_both(pkg, oracle, V2, rng.integers(1, len(V2) + 1, (500_000, 3)))
+
+
 def test_weld_batched_meshes(pkg, oracle):
     # every mesh of a mixed batch welds to what the definition gives
     rng = np.random.default_rng(5)
