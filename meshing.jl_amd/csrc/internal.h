@@ -50,9 +50,8 @@ struct Workspace {
   uint32_t* cnt;      // per unit: vertices | faces << 16
   int64_t* voff;      // per unit: exclusive prefix of vertices
   int64_t* foff;      // per unit: exclusive prefix of faces
-  uint64_t* status;   // per scan tile: look-back flag + aggregates
-  int64_t* incl;      // per scan tile: inclusive prefixes (v, f)
-  uint32_t* counter;  // scan tile ticket
+  uint64_t* tilesum;  // per scan tile: vertices | faces << 32
+  int64_t* tileoff;   // per scan tile: exclusive prefixes (v, f)
   int64_t* totals;    // (0, nverts, 0, nfaces): the sizes of a single volume
   double* coords;     // xp[nx], yp[ny], zp[nz] as double
 };

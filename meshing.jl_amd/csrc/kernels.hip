@@ -7,8 +7,8 @@
 //              (coalesced HBM reads); each lane marches z, keeping the
 //              previous sample plane in registers and taking the x+1
 //              neighbour from the next lane.
-//   k_scan     single-pass decoupled look-back scan of the unit counts in
-//              the reference's emission order (x outermost, z innermost).
+//   k_scan_*   exclusive prefix of the unit counts in the reference's
+//              emission order (x outermost, z innermost), reduce-then-scan.
 //   k_emit_*   re-read the field brick by brick into LDS, run each z-row
 //              chunk of 64 cells as one wavefront (lanes along z, the
 //              emission order), wave-scan the per-cell counts and write the
@@ -99,135 +99,111 @@ __global__ void k_sizes(const VolArgs* __restrict__ vols, int n, const int64_t* 
   sizes[4 * v + 3] = fe - fb;
 }
 
-// ---------------------------------------------------------------------------
-// Scan: exclusive prefix of the unit counts, single pass with decoupled
-// look-back.  status[t]: bits 62-63 flag (1 aggregate, 2 inclusive),
-// bits 0-30 tile vertex count, bits 31-61 tile face count.  incl[2t..2t+1]
-// hold the inclusive prefixes, published before the flag turns 2.
-__device__ __forceinline__ uint64_t ld_acquire(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int64_t ld_relaxed(const int64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_release(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_relaxed(int64_t* p, int64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ void k_zero_u64(uint64_t* __restrict__ p, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
 __global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan(const uint32_t* __restrict__ cnt, int64_t units,
-                                                 int64_t* __restrict__ voff,
-                                                 int64_t* __restrict__ foff,
-                                                 uint64_t* __restrict__ status,
-                                                 int64_t* __restrict__ incl,
-                                                 uint32_t* __restrict__ counter,
-                                                 int64_t* __restrict__ totals, int64_t ntiles) {
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_wsum[kWaves];
-  __shared__ int64_t s_excl[2];
-  if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-  __syncthreads();
-  const int64_t tile = s_tile;
-  const int lane = lane_id(), wv = wave_id();
-  const int64_t base = tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
+// ---------------------------------------------------------------------------
+// Scan: exclusive prefix of the unit counts (vertices in bits 0-15 of a
+// count, faces in bits 16-31) over tiles of kScanTile units, in three
+// launches that never wait on one another's blocks: k_scan_reduce sums each
+// tile, k_scan_tiles (one block) turns the tile sums into tile offsets and
+// the totals, k_scan_apply scans each tile from its offset.  A single-pass
+// scan with decoupled look-back took 1.1 ms over the 10.9 M units of a
+// 512-volume batch: every tile's inclusive prefix is a round trip through
+// memory that the next tiles wait on.
 
-  uint32_t c[kScanItems];
-  uint64_t sum = 0;  // vertices in the low 32 bits, faces in the high 32 bits
+// This thread's kScanItems counts and their packed sum (vertices in the low
+// 32 bits, faces in the high 32 bits; a tile's sums fit both halves).
+__device__ __forceinline__ uint64_t scan_load(const uint32_t* __restrict__ cnt, int64_t units,
+                                              int64_t base, uint32_t (&c)[kScanItems]) {
+  uint64_t sum = 0;
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     c[q] = base + q < units ? cnt[base + q] : 0u;
     sum += (uint64_t)(c[q] & 0xFFFF) | ((uint64_t)(c[q] >> 16) << 32);
   }
-  const uint64_t wincl = wave_incl_scan64(sum);
-  if (lane == kWave - 1) s_wsum[wv] = wincl;
-  __syncthreads();
-  uint64_t before = 0, agg = 0;
-#pragma unroll
-  for (int q = 0; q < kWaves; ++q) {
-    if (q < wv) before += s_wsum[q];
-    agg += s_wsum[q];
-  }
-  const uint64_t texcl = before + wincl - sum;
-  const int64_t agg_v = (int64_t)(agg & 0xFFFFFFFFu), agg_f = (int64_t)(agg >> 32);
+  return sum;
+}
 
-  if (wv == 0) {
-    int64_t ev = 0, ef = 0;
-    if (tile == 0) {
-      if (lane == 0) {
-        st_relaxed(&incl[0], agg_v);
-        st_relaxed(&incl[1], agg_f);
-        st_release(&status[0], 2ull << 62);
-      }
-    } else {
-      if (lane == 0)
-        st_release(&status[tile], (1ull << 62) | (uint64_t)agg_v | ((uint64_t)agg_f << 31));
-      int64_t j = tile - 1;
-      while (true) {
-        const int64_t t = j - lane;
-        uint64_t st = 0;
-        if (t >= 0) {
-          do {
-            st = ld_acquire(&status[t]);
-          } while ((st >> 62) == 0);
-        }
-        const bool incl_here = t >= 0 && (st >> 62) == 2;
-        const uint64_t ball = __ballot(incl_here);
-        int64_t cv = 0, cf = 0;
-        if (ball) {
-          const int first = __ffsll((unsigned long long)ball) - 1;
-          if (lane < first) {
-            cv = (int64_t)(st & 0x7FFFFFFFull);
-            cf = (int64_t)((st >> 31) & 0x7FFFFFFFull);
-          } else if (lane == first) {
-            cv = ld_relaxed(&incl[2 * t]);
-            cf = ld_relaxed(&incl[2 * t + 1]);
-          }
-        } else if (t >= 0) {
-          cv = (int64_t)(st & 0x7FFFFFFFull);
-          cf = (int64_t)((st >> 31) & 0x7FFFFFFFull);
-        }
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ cnt,
+                                                        int64_t units,
+                                                        uint64_t* __restrict__ tilesum) {
+  __shared__ uint64_t s_w[kWaves];
+  uint32_t c[kScanItems];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  const uint64_t incl = wave_incl_scan64(scan_load(cnt, units, base, c));
+  if (lane_id() == kWave - 1) s_w[wave_id()] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t agg = 0;
 #pragma unroll
-        for (int d = kWave / 2; d >= 1; d >>= 1) {
-          cv += __shfl_xor(cv, d, kWave);
-          cf += __shfl_xor(cf, d, kWave);
-        }
-        ev += cv;
-        ef += cf;
-        if (ball) break;
-        j -= kWave;
-      }
-      if (lane == 0) {
-        st_relaxed(&incl[2 * tile], ev + agg_v);
-        st_relaxed(&incl[2 * tile + 1], ef + agg_f);
-        st_release(&status[tile], 2ull << 62);
-      }
-    }
-    if (lane == 0) {
-      s_excl[0] = ev;
-      s_excl[1] = ef;
-      if (tile == ntiles - 1) {
-        totals[0] = 0;
-        totals[1] = ev + agg_v;
-        totals[2] = 0;
-        totals[3] = ef + agg_f;
-      }
-    }
+    for (int q = 0; q < kWaves; ++q) agg += s_w[q];
+    tilesum[blockIdx.x] = agg;
+  }
+}
+
+constexpr int kScanTilesBlock = 1024;
+
+__global__ __launch_bounds__(kScanTilesBlock) void k_scan_tiles(
+    const uint64_t* __restrict__ tilesum, int64_t ntiles, int64_t* __restrict__ tileoff,
+    int64_t* __restrict__ totals) {
+  __shared__ int64_t s_v[kScanTilesBlock / kWave], s_f[kScanTilesBlock / kWave];
+  const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
+  const int64_t per = (ntiles + kScanTilesBlock - 1) / kScanTilesBlock;
+  const int64_t lo = t * per, hi = lo + per < ntiles ? lo + per : ntiles;
+  int64_t sv = 0, sf = 0;  // int64: the sums over many tiles outgrow 32 bits
+  for (int64_t k = lo; k < hi; ++k) {
+    sv += (int64_t)(tilesum[k] & 0xFFFFFFFFu);
+    sf += (int64_t)(tilesum[k] >> 32);
+  }
+  const int64_t iv = (int64_t)wave_incl_scan64((uint64_t)sv);
+  const int64_t jf = (int64_t)wave_incl_scan64((uint64_t)sf);
+  if (lane == kWave - 1) {
+    s_v[wv] = iv;
+    s_f[wv] = jf;
   }
   __syncthreads();
-  int64_t rv = s_excl[0] + (int64_t)(texcl & 0xFFFFFFFFu);
-  int64_t rf = s_excl[1] + (int64_t)(texcl >> 32);
+  int64_t bv = 0, bf = 0;
+  for (int q = 0; q < wv; ++q) {
+    bv += s_v[q];
+    bf += s_f[q];
+  }
+  int64_t rv = bv + iv - sv, rf = bf + jf - sf;
+  for (int64_t k = lo; k < hi; ++k) {
+    tileoff[2 * k] = rv;
+    tileoff[2 * k + 1] = rf;
+    rv += (int64_t)(tilesum[k] & 0xFFFFFFFFu);
+    rf += (int64_t)(tilesum[k] >> 32);
+  }
+  if (t == kScanTilesBlock - 1) {
+    totals[0] = 0;
+    totals[1] = bv + iv;
+    totals[2] = 0;
+    totals[3] = bf + jf;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ cnt,
+                                                       int64_t units,
+                                                       const int64_t* __restrict__ tileoff,
+                                                       int64_t* __restrict__ voff,
+                                                       int64_t* __restrict__ foff) {
+  __shared__ uint64_t s_w[kWaves];
+  uint32_t c[kScanItems];
+  const int wv = wave_id();
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  const uint64_t sum = scan_load(cnt, units, base, c);
+  const uint64_t wincl = wave_incl_scan64(sum);
+  if (lane_id() == kWave - 1) s_w[wv] = wincl;
+  __syncthreads();
+  uint64_t before = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) before += q < wv ? s_w[q] : 0ull;
+  const uint64_t texcl = before + wincl - sum;  // within the tile: both halves fit
+  int64_t rv = tileoff[2 * blockIdx.x] + (int64_t)(texcl & 0xFFFFFFFFu);
+  int64_t rf = tileoff[2 * blockIdx.x + 1] + (int64_t)(texcl >> 32);
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     if (base + q < units) {
@@ -789,12 +765,12 @@ hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream
 
 hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st) {
   const int64_t nt = scan_tiles(g.units);
-  // status[nt] and the ticket counter are contiguous: status | counter
-  const int64_t nz = nt + 1;
-  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st,
-                     w.status, nz);
-  hipLaunchKernelGGL(k_scan, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt, g.units, w.voff,
-                     w.foff, w.status, w.incl, w.counter, w.totals, nt);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt, g.units,
+                     w.tilesum);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanTilesBlock), 0, st,
+                     (const uint64_t*)w.tilesum, nt, w.tileoff, w.totals);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt, g.units,
+                     (const int64_t*)w.tileoff, w.voff, w.foff);
   return hipGetLastError();
 }
 
