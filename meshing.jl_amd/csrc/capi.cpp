@@ -58,6 +58,8 @@ struct BatchState {
                                    // count launch, then 8 per emit launch
 };
 
+constexpr int kSpreadWays = 3;  // streams a batched phase's launches spread over
+
 struct Ctx {
   int device = 0;
   hipStream_t own = nullptr;
@@ -76,6 +78,10 @@ struct Ctx {
   size_t hbatch_bytes = 0;
   hipEvent_t upload = nullptr;  // last descriptor upload from hbatch
   bool upload_pending = false;
+  // batch: more streams (default priority) that a phase's launches spread
+  // over, forked from and joined back to `stream` by events
+  hipStream_t aux[kSpreadWays - 1] = {};
+  hipEvent_t fork = nullptr, join[kSpreadWays - 1] = {};
   int64_t* pinned = nullptr;  // 4 x int64 for the totals
   void* scratch = nullptr;    // this device's static scratch (msh_isosurface_noalloc)
   void* wws = nullptr;        // weld: hash table, scan workspace, indices
@@ -146,6 +152,51 @@ hipError_t run_phase(Ctx* c, PhaseGraph* g, std::vector<char> key, bool graphabl
   ++g->replays;
   return hipGraphLaunch(g->exec, c->stream);
 }
+
+// A phase's launches spread over up to kSpreadWays streams: the context's
+// and more, forked after the work queued so far and joined before anything
+// queued later.  Persistent launches of different kernels then run side by
+// side, so the CUs one launch leaves as its last tiles drain take the next
+// launch's tiles instead of idling; in a captured phase the streams become
+// parallel branches of the graph.  The launches share nothing but the
+// inputs they read.
+struct Spread {
+  Ctx* c;
+  int ways = 1;
+  std::vector<int64_t> load;
+  hipError_t start(int nlaunch) {
+    ways = nlaunch < kSpreadWays ? (nlaunch < 1 ? 1 : nlaunch) : kSpreadWays;
+    load.assign((size_t)ways, 0);
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < ways - 1 && e == hipSuccess; ++i) {
+      if (!c->aux[i]) e = hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking);
+      if (e == hipSuccess && !c->join[i])
+        e = hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming);
+    }
+    if (e == hipSuccess && ways > 1 && !c->fork)
+      e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (e == hipSuccess && ways > 1) e = hipEventRecord(c->fork, c->stream);
+    for (int i = 0; i < ways - 1 && e == hipSuccess; ++i)
+      e = hipStreamWaitEvent(c->aux[i], c->fork, 0);
+    return e;
+  }
+  // the stream with the least work so far takes a launch of `work` tiles
+  hipStream_t next(int64_t work) {
+    int k = 0;
+    for (int i = 1; i < ways; ++i)
+      if (load[i] < load[k]) k = i;
+    load[k] += work;
+    return k == 0 ? c->stream : c->aux[k - 1];
+  }
+  hipError_t finish() {
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < ways - 1 && e == hipSuccess; ++i) {
+      e = hipEventRecord(c->join[i], c->aux[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->join[i], 0);
+    }
+    return e;
+  }
+};
 
 int fail(Ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -440,6 +491,12 @@ int msh_destroy(void* ctx) {
   if (c->hbatch) (void)hipHostFree(c->hbatch);
   graph_drop(&c->gcount);
   graph_drop(&c->gemit);
+  for (int i = 0; i < kSpreadWays - 1; ++i) {
+    if (c->aux[i]) (void)hipStreamSynchronize(c->aux[i]);
+    if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
+    if (c->join[i]) (void)hipEventDestroy(c->join[i]);
+  }
+  if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->upload) (void)hipEventDestroy(c->upload);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own) (void)hipStreamDestroy(c->own);
@@ -779,9 +836,12 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
     if (e == hipSuccess) e = launch_fill_maps(dvols, n, cmap, emap, c->stream);
     if (e == hipSuccess) e = launch_coords_batch(dvols, n, c->stream);
     if (e == hipSuccess) e = launch_zero_u32(ctrs, (int64_t)(8 * b.counts.size()), c->stream);
+    Spread sp{c};
+    if (e == hipSuccess) e = sp.start((int)b.counts.size());
     for (size_t k = 0; k < b.counts.size() && e == hipSuccess; ++k)
       e = launch_count_batch(b.counts[k].plan, dvols, cmap, b.counts[k].gbase,
-                             b.counts[k].nblocks, ctrs + 8 * k, c->stream);
+                             b.counts[k].nblocks, ctrs + 8 * k, sp.next(b.counts[k].nblocks));
+    if (e == hipSuccess) e = sp.finish();
     Geom gall{};
     gall.units = units;
     if (e == hipSuccess && units > 0) e = launch_scan(gall, w, c->stream);
@@ -876,9 +936,12 @@ int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_p
                                   hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipEventRecord(c->upload, c->stream);
     if (e == hipSuccess) e = launch_zero_u32(ectr, (int64_t)(8 * b.emits.size()), c->stream);
+    Spread sp{c};
+    if (e == hipSuccess) e = sp.start((int)b.emits.size());
     for (size_t k = 0; k < b.emits.size() && e == hipSuccess; ++k)
       e = launch_emit_batch(b.emits[k].plan, b.emits[k].variant, dvols, emap, b.emits[k].gbase,
-                            b.emits[k].nblocks, ectr + 8 * k, c->stream);
+                            b.emits[k].nblocks, ectr + 8 * k, sp.next(b.emits[k].nblocks));
+    if (e == hipSuccess) e = sp.finish();
     return e;
   };
   std::vector<char> key;
