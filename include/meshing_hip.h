@@ -142,8 +142,11 @@ int msh_autotune(void* ctx, const msh_field* field, const msh_params* params, in
  * element type, size class), and each group's batched emit (all its volumes
  * in one msh_batch_extract, device outputs in a scratch buffer) is timed
  * with every family that takes all of them (a warm-up, then the best of 3);
- * the fastest is recorded for the group, and chosen[v] (n entries, may be
- * null) returns volume v's.  A batch runs the persistent batched kernels,
+ * the fastest is recorded for the group; then, since the groups' launches
+ * run side by side, one pass over the groups (most volumes first) times the
+ * whole batch with each other family that takes the group and keeps a
+ * change that makes it 1% faster.  chosen[v] (n entries, may be null)
+ * returns volume v's.  A batch runs the persistent batched kernels,
  * whose best shape differs from a lone small volume's (which cannot fill
  * the device by itself).  Leaves no batch state behind: call
  * msh_batch_count next. */

@@ -1133,6 +1133,76 @@ int msh_autotune_batch(void* ctx, int32_t n, const msh_field* fields, const msh_
     }
     c->family[method][T][cls] = best[g];
   }
+  // Then the whole batch: its groups' launches run side by side (Spread),
+  // so a group's fastest shape alone need not be fastest beside the others.
+  // One pass over the groups, most volumes first, keeps a change of one
+  // group's family when it takes the whole batch's emit 1% below the best so
+  // far.  took: every volume of group g takes family fam.
+  auto time_all = [&](int g, int fam, float* ms, bool* took) -> int {
+    std::vector<int64_t> nv((size_t)n), nf((size_t)n);
+    int r = msh_batch_count(ctx, n, fields, params, nv.data(), nf.data());
+    if (r) return r;
+    *took = true;
+    if (g >= 0)
+      for (int v : members[(size_t)g]) {
+        VolArgs probe = c->batch.vols[v];
+        *took = *took && variant_family(emit_variant(c->batch.plans[v], &probe)) == fam;
+      }
+    if (!*took) return MSH_OK;
+    size_t total = 256;
+    for (int v = 0; v < n; ++v)
+      total += align256((size_t)nv[v] * 3 * (c->batch.plans[v].Out == K_F32 ? 4 : 8)) +
+               align256((size_t)nf[v] * 24);
+    r = grow(c, &c->ostage, &c->ostage_bytes, total, "autotune output buffer");
+    if (r) return r;
+    std::vector<void*> vp((size_t)n);
+    std::vector<int64_t*> fp((size_t)n);
+    size_t off = 0;
+    for (int v = 0; v < n; ++v) {
+      vp[v] = (char*)c->ostage + off;
+      off += align256((size_t)nv[v] * 3 * (c->batch.plans[v].Out == K_F32 ? 4 : 8));
+      fp[v] = (int64_t*)((char*)c->ostage + off);
+      off += align256((size_t)nf[v] * 24);
+    }
+    for (int rep = 0; rep < 4; ++rep) {  // a warm-up, then the best of 3
+      hipError_t he = hipEventRecord(e0, c->stream);
+      if (he != hipSuccess) return hip_fail(c, he, "autotune");
+      r = msh_batch_extract(ctx, n, fields, params, vp.data(), fp.data(), MSH_DEVICE);
+      if (r) return r;
+      he = hipEventRecord(e1, c->stream);
+      if (he == hipSuccess) he = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (he == hipSuccess) he = hipEventElapsedTime(&t, e0, e1);
+      if (he != hipSuccess) return hip_fail(c, he, "autotune");
+      if (rep == 1 || (rep > 1 && t < *ms)) *ms = t;
+    }
+    return MSH_OK;
+  };
+  if (rc == MSH_OK && gkey.size() > 1) {
+    float cur = 0.f;
+    bool took = true;
+    rc = time_all(-1, 0, &cur, &took);
+    std::vector<size_t> gord(gkey.size());
+    for (size_t g = 0; g < gord.size(); ++g) gord[g] = g;
+    std::stable_sort(gord.begin(), gord.end(),
+                     [&](size_t x, size_t y) { return members[x].size() > members[y].size(); });
+    for (size_t g : gord) {
+      const int method = gkey[g] / (3 * kSizeClasses), T = (gkey[g] / kSizeClasses) % 3,
+                cls = gkey[g] % kSizeClasses;
+      for (int fam = 3; fam >= 0 && rc == MSH_OK; --fam) {
+        if (fam == best[g]) continue;
+        c->family[method][T][cls] = fam;
+        float ms = 0.f;
+        rc = time_all((int)g, fam, &ms, &took);
+        if (rc == MSH_OK && took && ms < 0.99f * cur) {
+          best[g] = fam;
+          cur = ms;
+        }
+      }
+      c->family[method][T][cls] = best[g];
+      if (rc) break;
+    }
+  }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   c->batch.have = false;  // the timing batches are not the caller's
