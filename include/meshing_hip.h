@@ -104,7 +104,11 @@ int msh_extract(void* ctx, const msh_field* field, const msh_params* params,
  * arrays, writes volume v into verts[v] and faces[v] (faces 1-based within
  * the volume).  Fields may be host or device memory; outputs as out_memory
  * says.  A batch shares the context's workspace with the single-volume calls:
- * msh_count in between invalidates it (MSH_ERR_STATE). */
+ * msh_count in between invalidates it (MSH_ERR_STATE).  The count and emit
+ * launches of a batch spread over the context's stream and up to two more
+ * that the context creates (default priority), forked from and joined back
+ * to its stream by events, so to the caller the batch stays ordered on that
+ * one stream. */
 int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,
                     int64_t* nverts, int64_t* nfaces);
 int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_params* params,

@@ -162,7 +162,8 @@ hipError_t run_phase(Ctx* c, PhaseGraph* g, std::vector<char> key, bool graphabl
 // inputs they read.
 struct Spread {
   Ctx* c;
-  int ways = 1;
+  int ways = 1;    // streams launches go to
+  int forked = 0;  // aux streams that waited on the fork
   std::vector<int64_t> load;
   hipError_t start(int nlaunch) {
     ways = nlaunch < kSpreadWays ? (nlaunch < 1 ? 1 : nlaunch) : kSpreadWays;
@@ -176,8 +177,11 @@ struct Spread {
     if (e == hipSuccess && ways > 1 && !c->fork)
       e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e == hipSuccess && ways > 1) e = hipEventRecord(c->fork, c->stream);
-    for (int i = 0; i < ways - 1 && e == hipSuccess; ++i)
+    for (int i = 0; i < ways - 1 && e == hipSuccess; ++i) {
       e = hipStreamWaitEvent(c->aux[i], c->fork, 0);
+      if (e == hipSuccess) forked = i + 1;
+    }
+    if (e != hipSuccess) ways = 1 + forked;  // launch only on streams that forked
     return e;
   }
   // the stream with the least work so far takes a launch of `work` tiles
@@ -188,13 +192,17 @@ struct Spread {
     load[k] += work;
     return k == 0 ? c->stream : c->aux[k - 1];
   }
+  // joins every stream the fork reached, also after a failed launch, so a
+  // capture in progress always ends joined
   hipError_t finish() {
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < ways - 1 && e == hipSuccess; ++i) {
-      e = hipEventRecord(c->join[i], c->aux[i]);
+    hipError_t first = hipSuccess;
+    for (int i = 0; i < forked; ++i) {
+      hipError_t e = hipEventRecord(c->join[i], c->aux[i]);
       if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->join[i], 0);
+      if (first == hipSuccess) first = e;
     }
-    return e;
+    forked = 0;
+    return first;
   }
 };
 
@@ -841,7 +849,8 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
     for (size_t k = 0; k < b.counts.size() && e == hipSuccess; ++k)
       e = launch_count_batch(b.counts[k].plan, dvols, cmap, b.counts[k].gbase,
                              b.counts[k].nblocks, ctrs + 8 * k, sp.next(b.counts[k].nblocks));
-    if (e == hipSuccess) e = sp.finish();
+    const hipError_t ej = sp.finish();
+    if (e == hipSuccess) e = ej;
     Geom gall{};
     gall.units = units;
     if (e == hipSuccess && units > 0) e = launch_scan(gall, w, c->stream);
@@ -941,7 +950,8 @@ int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_p
     for (size_t k = 0; k < b.emits.size() && e == hipSuccess; ++k)
       e = launch_emit_batch(b.emits[k].plan, b.emits[k].variant, dvols, emap, b.emits[k].gbase,
                             b.emits[k].nblocks, ectr + 8 * k, sp.next(b.emits[k].nblocks));
-    if (e == hipSuccess) e = sp.finish();
+    const hipError_t ej = sp.finish();
+    if (e == hipSuccess) e = ej;
     return e;
   };
   std::vector<char> key;
