@@ -2,7 +2,7 @@
 """Benchmark: plain Marching Cubes on a 1024^3 value-noise field, one MI355X.
 
 A step is one full isosurface extraction of a field already resident in HBM:
-msh_count (classify + count + decoupled look-back scan + the 16-byte size
+msh_count (classify + count + reduce-then-scan + the 16-byte size
 read-back) followed by msh_extract into preallocated device buffers, i.e.
 everything the drop-in `isosurface(A, MarchingCubes(iso=0))` does except
 allocating its result arrays.
@@ -10,7 +10,9 @@ allocating its result arrays.
 Workload (default): Float64 1024^3 value noise (32^3 lattice of seeded
 Uniform(-1,1) values, trilinearly upsampled, plus 0.25 x a 128^3 octave),
 MarchingCubes(iso=0), X=Y=Z=-1:1.  The same field in Float32 is measured in
-the same run and reported under "float32".
+the same run and reported under "float32"; "sphere_1024_f32" is the sparse
+point (Float32 sphere of radius 0.5, few outputs) and "gyroid_256_f32" the
+Marching Tetrahedra / surface nets / Marching Cubes workloads.
 
 Printed: one JSON line.  `value` is Gvoxels/s (samples per second / 1e9).
 `roofline` is for the dominant kernel (the Marching Cubes emit kernel,
@@ -532,40 +534,59 @@ def summarize_counters(fetch_dir, write_dir, line_file, out_file=TRAFFIC_FILE, c
     for counter, rs in (("FETCH_SIZE", fetch), ("WRITE_SIZE", write)):
         cp = [float(r["Counter_Value"]) for r in rs if "copyBuffer" in r["Kernel_Name"]][-4:]
         scale[counter] = (copy_bytes / 1024.0) / (sum(cp) / len(cp))
-    loads = {"mc_value_noise_1024_float64": ("mc", "float64", line["float64"], max),
-             "mc_value_noise_1024_float32": ("mc", "float32", line["float32"], max)}
+    # the workloads in the order the bench measures them
+    loads = [("mc_value_noise_1024_float64", "mc", "float64", line["float64"], max),
+             ("mc_value_noise_1024_float32", "mc", "float32", line["float32"], max)]
     for meth, g in line.get("gyroid_256_f32", {}).items():
-        loads[f"{meth}_gyroid_256_float32"] = (meth, "float32", g, min)
+        loads.append((f"{meth}_gyroid_256_float32", meth, "float32", g, min))
+    for k, g in line.items():
+        if k.startswith("sphere_") and k.endswith("_f32") and isinstance(g, dict):
+            loads.append((f"mc_{k[:-4]}_float32", "mc", "float32", g, max))
     out = {"note": "HBM bytes per emit launch (FETCH_SIZE and WRITE_SIZE, KiB per dispatch, "
                    "averaged over the dispatches of the instantiation, times the scale a "
                    "1 GiB device copy gives each counter in the same run)",
            "scale": {k: round(v, 4) for k, v in scale.items()}, "workloads": {}}
-    for key, (meth, dtype, r, pick) in loads.items():
-        fam = r["family_id"]
-        pat = _emit_pattern(meth, dtype, fam)
-        per = {}
-        for counter, rs in (("FETCH_SIZE", fetch), ("WRITE_SIZE", write)):
+    per = [{} for _ in loads]
+    for counter, rs in (("FETCH_SIZE", fetch), ("WRITE_SIZE", write)):
+        groups = {}  # (kernel, grid) -> workloads resolving to it, in bench order
+        found = {}
+        for i, (key, meth, dtype, r, pick) in enumerate(loads):
+            pat = _emit_pattern(meth, dtype, r["family_id"])
             grids = {}
             for x in rs:
                 if pat.search(x["Kernel_Name"]):
                     grids.setdefault((x["Kernel_Name"], int(x["Grid_Size"])), []).append(
-                        float(x["Counter_Value"]))
+                        (int(x["Dispatch_Id"]), float(x["Counter_Value"])))
             if not grids:
-                break
+                continue
             # the 1024^3 launches have the largest grid of their instantiation,
             # the 256^3 ones the smallest
-            kname, grid = pick(grids, key=lambda k: k[1])
-            vals = grids[(kname, grid)]
-            per[counter] = dict(kernel=kname, grid=grid, dispatches=len(vals),
-                                kib=sum(vals) / len(vals))
-        if len(per) < 2:
+            g = pick(grids, key=lambda k: k[1])
+            found[g] = sorted(grids[g])
+            groups.setdefault(g, []).append(i)
+        for g, idxs in groups.items():
+            ds = found[g]
+            # workloads on the same instantiation and grid (Float32 value noise
+            # and the sphere at 1024^3) ran one after the other: split the
+            # dispatches at the widest gaps in dispatch id, many other
+            # workloads' dispatches lie between them
+            cuts = sorted(sorted(range(1, len(ds)), key=lambda j: ds[j][0] - ds[j - 1][0],
+                                 reverse=True)[:len(idxs) - 1])
+            parts = [ds[lo:hi] for lo, hi in zip([0] + cuts, cuts + [len(ds)])]
+            for i, part in zip(idxs, parts):
+                if part:
+                    per[i][counter] = dict(kernel=g[0], grid=g[1], dispatches=len(part),
+                                           kib=sum(v for _, v in part) / len(part))
+    for (key, meth, dtype, r, pick), pc in zip(loads, per):
+        if len(pc) < 2:
             continue
-        fb = per["FETCH_SIZE"]["kib"] * 1024 * scale["FETCH_SIZE"]
-        wb = per["WRITE_SIZE"]["kib"] * 1024 * scale["WRITE_SIZE"]
+        fam = r["family_id"]
+        fb = pc["FETCH_SIZE"]["kib"] * 1024 * scale["FETCH_SIZE"]
+        wb = pc["WRITE_SIZE"]["kib"] * 1024 * scale["WRITE_SIZE"]
         alg = r.get("algorithmic_bytes") or r["roofline"]["algorithmic_bytes_per_launch"]
         out["workloads"][key] = dict(
-            kernel=per["FETCH_SIZE"]["kernel"], grid=per["FETCH_SIZE"]["grid"],
-            family_id=fam, dispatches=per["FETCH_SIZE"]["dispatches"],
+            kernel=pc["FETCH_SIZE"]["kernel"], grid=pc["FETCH_SIZE"]["grid"],
+            family_id=fam, dispatches=pc["FETCH_SIZE"]["dispatches"],
             fetch_bytes=int(fb), write_bytes=int(wb), bytes_per_launch=int(fb + wb),
             algorithmic_bytes=alg, traffic_over_algorithmic=round((fb + wb) / alg, 3))
     Path(out_file).write_text(json.dumps(out, indent=1) + "\n")
@@ -648,6 +669,17 @@ def main():
                             emit_ms=round(r["emit_s"] * 1e3, 4),
                             nverts=r["nverts"], nfaces=r["nfaces"],
                             roofline=roofline(r, meth, f"{meth}_gyroid_256_float32"))
+    # the sparse 1024^3 point: Float32 sphere of radius 0.5 (few outputs, so
+    # the reads dominate)
+    sph = measure(m, "float32", a.n, "mc", a.steps, a.warmup, kind="sphere")
+    sphere = dict(gvoxels_s=round(sph["gvox_s"], 3), tris_s=round(sph["tris_s"], 1),
+                  autotuned_emit_kernels=sph["family"], family_id=sph["family_id"],
+                  ms_per_step=round(sph["step_s"] * 1e3, 4),
+                  count_ms=round(sph["count_s"] * 1e3, 4), emit_ms=round(sph["emit_s"] * 1e3, 4),
+                  nverts=sph["nverts"], nfaces=sph["nfaces"],
+                  step_hbm_frac=round((sph["field_bytes"] + sph["out_bytes"]) / sph["step_s"]
+                                      / HBM_PEAK, 4),
+                  roofline=roofline(sph, "mc", f"mc_sphere_{a.n}_float32"))
     batch = measure_batch(m, a.batch_volumes, max(2, min(a.steps, 5))) if a.batch_volumes else None
     serve = (measure_serve(m, a.serve_requests, [float(q) for q in a.serve_qps.split(",")])
              if a.serve_requests else None)
@@ -691,6 +723,7 @@ def main():
         "hbm_copy_probe": copy,
         "drop_in_host_arrays": host,
         "gyroid_256_f32": gyroid,
+        f"sphere_{a.n}_f32": sphere,
         "batched": batch,
         "serving": serve,
         "vertex_error_float32_vs_float64": verr,

@@ -1147,7 +1147,9 @@ int msh_autotune_batch(void* ctx, int32_t n, const msh_field* fields, const msh_
   // so a group's fastest shape alone need not be fastest beside the others.
   // One pass over the groups, most volumes first, keeps a change of one
   // group's family when it takes the whole batch's emit 1% below the best so
-  // far.  took: every volume of group g takes family fam.
+  // far (skipped when the device has no room for all the outputs at once).
+  // took: every volume of group g takes family fam.
+  bool no_room = false;
   auto time_all = [&](int g, int fam, float* ms, bool* took) -> int {
     std::vector<int64_t> nv((size_t)n), nf((size_t)n);
     int r = msh_batch_count(ctx, n, fields, params, nv.data(), nf.data());
@@ -1164,7 +1166,13 @@ int msh_autotune_batch(void* ctx, int32_t n, const msh_field* fields, const msh_
       total += align256((size_t)nv[v] * 3 * (c->batch.plans[v].Out == K_F32 ? 4 : 8)) +
                align256((size_t)nf[v] * 24);
     r = grow(c, &c->ostage, &c->ostage_bytes, total, "autotune output buffer");
-    if (r) return r;
+    if (r) {  // no room for the whole batch's outputs: the group choices stand
+      (void)hipGetLastError();
+      c->err.clear();
+      *took = false;
+      no_room = true;
+      return MSH_OK;
+    }
     std::vector<void*> vp((size_t)n);
     std::vector<int64_t*> fp((size_t)n);
     size_t off = 0;
@@ -1192,7 +1200,8 @@ int msh_autotune_batch(void* ctx, int32_t n, const msh_field* fields, const msh_
     float cur = 0.f;
     bool took = true;
     rc = time_all(-1, 0, &cur, &took);
-    std::vector<size_t> gord(gkey.size());
+    // (with no room for the whole batch there is nothing to compare against)
+    std::vector<size_t> gord(rc == MSH_OK && !no_room ? gkey.size() : 0);
     for (size_t g = 0; g < gord.size(); ++g) gord[g] = g;
     std::stable_sort(gord.begin(), gord.end(),
                      [&](size_t x, size_t y) { return members[x].size() > members[y].size(); });
@@ -1204,13 +1213,14 @@ int msh_autotune_batch(void* ctx, int32_t n, const msh_field* fields, const msh_
         c->family[method][T][cls] = fam;
         float ms = 0.f;
         rc = time_all((int)g, fam, &ms, &took);
+        if (no_room) break;
         if (rc == MSH_OK && took && ms < 0.99f * cur) {
           best[g] = fam;
           cur = ms;
         }
       }
       c->family[method][T][cls] = best[g];
-      if (rc) break;
+      if (rc || no_room) break;
     }
   }
   if (e0) (void)hipEventDestroy(e0);

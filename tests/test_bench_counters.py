@@ -54,3 +54,32 @@ def test_summary_selects_timed_instantiation(tmp_path):
     # a summary from another family is not used for the timed kernel
     bench_t = json.loads((tmp_path / "t.json").read_text())
     assert bench_t["workloads"]["mc_value_noise_1024_float32"]["family_id"] == 2
+
+
+def test_summary_splits_workloads_on_one_instantiation(tmp_path):
+    # Float32 value noise and the Float32 sphere at 1024^3 run the same
+    # instantiation at the same grid, one after the other with the gyroid
+    # workloads between them: each takes its own run of dispatches
+    f32 = "msh::(anonymous namespace)::k_emit_mc5<float, float, float, 16, true, 0, 2>(msh::VolArgs)"
+    f64 = "msh::(anonymous namespace)::k_emit_mc5<double, double, double, 8, true, 0, 2>(msh::VolArgs)"
+    other = "msh::(anonymous namespace)::k_generate<float>(long)"
+    copy = "__amd_rocclr_copyBuffer"
+
+    def rows(vn, sph, kib_of_copy):
+        r = [(f64, 1000, 1.0)] + [(f32, 900, vn)] * 3 + [(other, 7, 0.0)] * 40 + \
+            [(f32, 900, sph)] * 2
+        return r + [(copy, 1, kib_of_copy)] * 4
+
+    _csv(tmp_path / "f", "FETCH_SIZE", rows(40.0, 20.0, 524288.0))
+    _csv(tmp_path / "w", "WRITE_SIZE", rows(30.0, 1.0, 1048576.0))
+    line = {"float64": {"family_id": 2, "algorithmic_bytes": 1000},
+            "float32": {"family_id": 2, "algorithmic_bytes": 1000},
+            "sphere_1024_f32": {"family_id": 2, "roofline": {"algorithmic_bytes_per_launch": 500}}}
+    (tmp_path / "line.json").write_text(json.dumps(line) + "\n")
+    out = bench.summarize_counters(tmp_path / "f", tmp_path / "w", tmp_path / "line.json",
+                                   out_file=tmp_path / "t.json")
+    w = out["workloads"]
+    vn, sp = w["mc_value_noise_1024_float32"], w["mc_sphere_1024_float32"]
+    assert vn["dispatches"] == 3 and sp["dispatches"] == 2
+    assert vn["fetch_bytes"] == 40 * 1024 * 2 and sp["fetch_bytes"] == 20 * 1024 * 2
+    assert vn["write_bytes"] == 30 * 1024 and sp["write_bytes"] == 1 * 1024

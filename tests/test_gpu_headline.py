@@ -121,6 +121,29 @@ def test_headline_float32_matches_float64_of_same_samples(pkg):
     torch.cuda.empty_cache()
 
 
+def test_sparse_sphere_1024_float32(pkg, oracle):
+    # the sparse 1024^3 point of the benchmark: Float32 sphere of radius 0.5
+    # on [-1, 1]^3; the middle x-slab (through the sphere's widest part)
+    # against the oracle, its offsets from counting the planes before it
+    import torch
+    print(f"sparse sphere: Float32 {N}^3, MarchingCubes(iso=0)", flush=True)
+    f = pkg.generate_field("sphere", (N, N, N), "float32", p0=0.5)
+    V, F = pkg.isosurface(f, pkg.MarchingCubes(iso=0))
+    assert V.dtype == torch.float32 and 10 ** 6 < len(F) < 10 ** 7
+    x0 = 509
+    nv0, nf0 = pkg.count(f[:x0], pkg.MarchingCubes(iso=0))
+    Vo, Fo = slab_cells(oracle, f, x0, x0 + SLAB - 1)
+    assert len(Fo) > 10000
+    assert_same_mesh(V[nv0:nv0 + len(Vo)].cpu().numpy(),
+                     (F[nf0:nf0 + len(Fo)] - nv0).cpu().numpy(), Vo, Fo)
+    # every vertex within half a cell of the sphere (the surface crosses
+    # each cut edge, and an edge is one cell long)
+    r = V.double().norm(dim=1)
+    assert float((r - 0.5).abs().max()) <= 1.0 / (N - 1)
+    del f, V, F, r
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_value_noise_256_against_oracle(pkg, oracle, dtype):
     name = "float64" if dtype == np.float64 else "float32"
