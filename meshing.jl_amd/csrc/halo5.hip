@@ -157,11 +157,17 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
     const bool emits = exists && lane >= 4 && lane < 4 + kH5TX;
     const bool hexists = wv == 0 && colok && y0 >= 0;
     int64_t rbv = 0, rbf = 0, hrbv = 0;
+    bool writes = false;
     if (exists) {
       const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + gj) * g.nzseg + s;
       rbv = a.voff[u] - vbase;
       rbf = a.foff[u] - fbase;
+      writes = emits && a.cnt[u] != 0;
     }
+    // a tile whose emitting rows the count found empty writes nothing (the
+    // rows around it only supply owners to faces): skip it before fetching
+    // any of its bricks
+    if (!__syncthreads_or(writes)) return;
     if (hexists) hrbv = a.voff[a.ubase + ((int64_t)gi * (g.ny - 1) + y0) * g.nzseg + s] - vbase;
 
     // DMA: planes zc-1 .. zc+ZC, rows y0 .. y0+4, 64 samples from x0.  Near

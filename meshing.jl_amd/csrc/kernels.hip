@@ -294,6 +294,7 @@ __device__ __forceinline__ void emit_tile(const VolArgs& a, int64_t t) {
     }
   }
   if (METHOD != MT && threadIdx.x < 12) sm.lut_e[threadIdx.x] = MC_EDGE[threadIdx.x];
+  bool any = false;  // some row of the brick, halo rows included, has outputs
   for (int r = threadIdx.x; r < NR; r += kBlock) {
     const int i = x0 + r % RX, j = y0 + r / RX;
     int64_t bv = 0, bf = 0;
@@ -301,10 +302,14 @@ __device__ __forceinline__ void emit_tile(const VolArgs& a, int64_t t) {
       const int64_t u = a.ubase + ((int64_t)i * (g.ny - 1) + j) * g.nzseg + s;
       bv = a.voff[u] - vbase;
       bf = a.foff[u] - fbase;
+      any = any || a.cnt[u] != 0;
     }
     sm.rbv[0][r] = bv;
     sm.rbf[0][r] = bf;
   }
+  // a tile whose rows the count found empty writes nothing: skip it
+  // before fetching any of its bricks (sparse fields have many)
+  if (!__syncthreads_or(any)) return;
   for (int q = threadIdx.x; q < SX; q += kBlock) sm.cx[q] = xp[min(max(x0 + q, 0), g.nx - 1)];
   for (int q = threadIdx.x; q < SY; q += kBlock) sm.cy[q] = yp[min(max(y0 + q, 0), g.ny - 1)];
 

@@ -293,6 +293,7 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
                          ((uint32_t)(mc_dx(ca) | (mc_dy(ca) << 1) | (mc_dz(ca) << 2)) << 20) |
                          ((uint32_t)(mc_dx(cb) | (mc_dy(cb) << 1) | (mc_dz(cb) << 2)) << 23);
   }
+  bool emits = false;
   for (int rr = threadIdx.x; rr < NRW; rr += kBlock) {
     const int ci = rr % TX, cj = rr / TX;
     int64_t bv = 0, bf = 0;
@@ -300,10 +301,14 @@ __device__ __forceinline__ void emit_mc4_tile(const VolArgs& a, int64_t t) {
       const int64_t u = a.ubase + ((int64_t)(ib + ci) * (g.ny - 1) + jb + cj) * g.nzseg + s;
       bv = a.voff[u] - vbase;
       bf = a.foff[u] - fbase;
+      emits = emits || a.cnt[u] != 0;
     }
     sm.rbv[0][rr] = bv;
     sm.rbf[0][rr] = bf;
   }
+  // a tile whose rows the count found empty writes nothing: skip it
+  // before fetching any of its bricks (sparse fields have many)
+  if (!__syncthreads_or(emits)) return;
   for (int c = threadIdx.x; c < SX; c += kBlock) sm.cx[c] = a.coords[min(ib + c, g.nx - 1)];
   if (threadIdx.x < SY) sm.cy[threadIdx.x] = a.coords[g.nx + min(jb + (int)threadIdx.x, g.ny - 1)];
 

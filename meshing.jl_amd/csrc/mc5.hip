@@ -101,11 +101,16 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
     // this lane's row (ib + lane, jb + wv): output offsets, carried chunk to chunk
     const bool rowok = lane < TX && ib + lane < g.nx - 1 && jb + wv < g.ny - 1;
     int64_t rbv = 0, rbf = 0;
+    bool emits = false;
     if (rowok) {
       const int64_t u = a.ubase + ((int64_t)(ib + lane) * (g.ny - 1) + jb + wv) * g.nzseg + s;
       rbv = a.voff[u] - vbase;
       rbf = a.foff[u] - fbase;
+      emits = a.cnt[u] != 0;
     }
+    // a tile whose rows the count found empty writes nothing: skip it
+    // before fetching any of its bricks (sparse fields have many)
+    if (!__syncthreads_or(emits)) return;
 
     // DMA layout (as in the emit kernel with lanes along z): per-lane byte
     // offsets relative to the chunk's first sample are fixed.

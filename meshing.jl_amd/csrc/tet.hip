@@ -130,6 +130,7 @@ __device__ __forceinline__ void emit_halo_tile(const VolArgs& a, int64_t t) {
   for (int q = threadIdx.x; q < SX; q += kBlock)
     sm.cx[q] = a.coords[min(max(x0 + q, 0), g.nx - 1)];
   if (threadIdx.x < SY) sm.cy[threadIdx.x] = a.coords[g.nx + min(max(y0 + (int)threadIdx.x, 0), g.ny - 1)];
+  bool any = false;  // some row of the brick, halo rows included, has outputs
   for (int q = threadIdx.x; q < kWaves * WR; q += kBlock) {
     const int w = q / WR, r = q % WR;
     const int xr = r & 15, yr = r >> 4;
@@ -139,10 +140,14 @@ __device__ __forceinline__ void emit_halo_tile(const VolArgs& a, int64_t t) {
       const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + gj) * g.nzseg + s;
       bv = a.voff[u] - vbase;
       bf = a.foff[u] - fbase;
+      any = any || a.cnt[u] != 0;
     }
     sm.rbv[0][w][r] = bv;
     sm.rbf[0][w][r] = bf;
   }
+  // a tile whose rows the count found empty writes nothing: skip it
+  // before fetching any of its bricks (sparse fields have many)
+  if (!__syncthreads_or(any)) return;
 
   // DMA: planes zc-1 .. zc+ZC, rows y0 .. y0+2, 64 samples from x0.  Per-lane
   // byte offsets relative to the brick origin are fixed; near the volume's

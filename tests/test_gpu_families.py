@@ -40,6 +40,34 @@ def test_family_matches_oracle(pkg, oracle, reset_families, family, name, dtype)
         assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
 
 
+def sparse_field(dtype):
+    # a small sphere off centre in a larger ragged box: most emit tiles of
+    # every family hold no active cell
+    x, y, z = np.meshgrid(np.linspace(-1, 1, 150), np.linspace(-1, 1, 70),
+                          np.linspace(-1, 1, 90), indexing="ij")
+    return (np.sqrt((x - 0.4) ** 2 + (y + 0.3) ** 2 + z * z) - 0.22).astype(dtype)
+
+
+@pytest.mark.parametrize("family", [3, 2, 1, 0])
+@pytest.mark.parametrize("name,dtype", CASES)
+def test_family_skips_empty_tiles(pkg, oracle, reset_families, family, name, dtype):
+    # tiles whose rows the count found empty are skipped whole; the meshes
+    # (alone, and in a batch with a volume nothing crosses) do not change
+    m = method_of(pkg, name)
+    pkg.set_kernel_family(m, dtype, family)
+    f = sparse_field(dtype)
+    empty = np.ones((40, 33, 20), dtype)
+    Vo, Fo = oracle.isosurface(f, name, iso=0)
+    assert len(Fo) > 1000
+    V, F = pkg.isosurface(dev_field(f), m)
+    assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
+    res = pkg.isosurface_batch([dev_field(f), dev_field(empty), dev_field(f[::-1].copy())], m)
+    assert len(res[1][1]) == 0 and len(res[1][0]) == 0
+    assert_same_mesh(res[0][0].cpu().numpy(), res[0][1].cpu().numpy(), Vo, Fo)
+    Vr, Fr = oracle.isosurface(f[::-1].copy(), name, iso=0)
+    assert_same_mesh(res[2][0].cpu().numpy(), res[2][1].cpu().numpy(), Vr, Fr)
+
+
 @pytest.mark.parametrize("name,dtype", CASES)
 def test_autotune_keeps_results(pkg, oracle, reset_families, name, dtype):
     m = method_of(pkg, name)
