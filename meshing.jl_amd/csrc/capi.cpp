@@ -374,16 +374,18 @@ size_t workspace_layout(int64_t units, int64_t ncoords, void* base, Workspace* w
   const int64_t U = units > 0 ? units : 1;
   const int64_t nt = scan_tiles(U);
   const size_t b_cnt = align256((size_t)U * 4);
-  const size_t b_off = align256((size_t)U * 8);
+  const size_t b_off = align256((size_t)U * 8);  // also the z masks
   const size_t b_tsum = align256((size_t)nt * 8);
   const size_t b_toff = align256((size_t)nt * 16);
   const size_t b_tot = 256;
   const size_t b_coords = align256((size_t)(ncoords + 1) * 8);
-  const size_t need = b_cnt + 2 * b_off + b_tsum + b_toff + b_tot + b_coords;
+  const size_t need = b_cnt + 3 * b_off + b_tsum + b_toff + b_tot + b_coords;
   if (!base || !w) return need;
   char* p = (char*)base;
   w->cnt = (uint32_t*)p;
   p += b_cnt;
+  w->zmask = (uint64_t*)p;
+  p += b_off;
   w->voff = (int64_t*)p;
   p += b_off;
   w->foff = (int64_t*)p;

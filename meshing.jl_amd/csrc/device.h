@@ -56,6 +56,35 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// The OR over the block of every thread's 64-bit value, with one barrier
+// (`slot`: kWaves words of LDS, free until every wave has passed the
+// barrier that starts the next tile): a tile's z-block activity mask from
+// its rows' masks.
+__device__ __forceinline__ uint64_t tile_zmask(uint64_t m, uint64_t* slot) {
+  for (int d = kWave / 2; d >= 1; d >>= 1) m |= __shfl_xor(m, d, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) slot[threadIdx.x / kWave] = m;
+  lds_barrier();
+  uint64_t t = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) t |= slot[w];
+  return t;
+}
+
+// The first chunk after the one at zc (chunks of zcl cells from zs, which
+// never straddle a z block of 1 << zsh cells) whose block is marked in
+// tmask, or ze when there is none.
+__device__ __forceinline__ int next_zchunk(int zc, int zcl, int zs, int ze, int zsh,
+                                           uint64_t tmask) {
+  const int nz = zc + zcl;
+  if (nz >= ze) return ze;
+  const int b = (nz - zs) >> zsh;
+  if ((tmask >> b) & 1) return nz;
+  const uint64_t rest = b < 63 ? tmask >> (b + 1) : 0ull;
+  if (rest == 0) return ze;
+  const int zn = zs + ((b + 1 + __builtin_ctzll(rest)) << zsh);
+  return zn < ze ? zn : ze;
+}
+
 // Lane i gets lane i+1's value (DPP wave_shl:1); lane 63 gets `last`.
 __device__ __forceinline__ uint32_t next_lane(uint32_t v, uint32_t last) {
   return (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)v, 0x130, 0xF, 0xF, false);

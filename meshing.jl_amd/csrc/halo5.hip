@@ -64,6 +64,7 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
     // queue entries: x | z << 6 | case << 10 | class << 18 | first slot << 21
     uint32_t vlist[kWaves][CAPE], flist[kWaves][CAPE];
     uint64_t vbits[kWaves][CAPV / 64], fbits[kWaves][CAPF / 64];
+    uint64_t tzm[kWaves];  // the tile's z-block mask, per wave
   };
   __shared__ Smem sm;
   const int lane = lane_id(), wv = wave_id();
@@ -158,17 +159,30 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
     const bool hexists = wv == 0 && colok && y0 >= 0;
     int64_t rbv = 0, rbf = 0, hrbv = 0;
     bool writes = false;
+    uint64_t zm = 0;
     if (exists) {
       const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + gj) * g.nzseg + s;
       rbv = a.voff[u] - vbase;
       rbf = a.foff[u] - fbase;
       writes = emits && a.cnt[u] != 0;
+      zm = a.cnt[u] != 0 ? a.zmask[u] : 0ull;
     }
     // a tile whose emitting rows the count found empty writes nothing (the
     // rows around it only supply owners to faces): skip it before fetching
     // any of its bricks
     if (!__syncthreads_or(writes)) return;
-    if (hexists) hrbv = a.voff[a.ubase + ((int64_t)gi * (g.ny - 1) + y0) * g.nzseg + s] - vbase;
+    if (hexists) {
+      const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + y0) * g.nzseg + s;
+      hrbv = a.voff[u] - vbase;
+      zm |= a.cnt[u] != 0 ? a.zmask[u] : 0ull;
+    }
+    // the z blocks where some row of the brick, the rows around it included,
+    // may hold an active cell: the chunks of the other blocks change no
+    // row's vertex offset, write nothing and are never fetched
+    const uint64_t tmask = tile_zmask(zm, sm.tzm);
+    if (tmask == 0) return;  // (an emitting row with outputs marks a block)
+    const int zsh = zmask_shift(g.zseg);
+    const int z0 = zs + (__builtin_ctzll(tmask) << zsh);
 
     // DMA: planes zc-1 .. zc+ZC, rows y0 .. y0+4, 64 samples from x0.  Near
     // the volume's low faces the origin is outside the field: the buffer starts
@@ -228,14 +242,15 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
     for (int q = lane; q < CAPV / 64; q += kWave) vbits[q] = 0;
     for (int q = lane; q < CAPF / 64; q += kWave) fbits[q] = 0;
 
-    if (zs < ze) fetch(zs, 0);
+    if (z0 < ze) fetch(z0, 0);
     __builtin_amdgcn_s_waitcnt(0);
     lds_barrier();
     int buf = 0;
-    for (int zc = zs; zc < ze; zc += ZC, buf ^= NBUF - 1) {
+    for (int zc = z0, znx; zc < ze; zc = znx, buf ^= NBUF - 1) {
+      znx = next_zchunk(zc, ZC, zs, ze, zsh, tmask);
       // double-buffered: the next brick streams in during this chunk;
       // single-buffered: it is fetched after the chunk (more blocks per CU)
-      if (NBUF == 2 && zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
+      if (NBUF == 2 && znx < ze) fetch(znx, buf ^ 1);
       const uint32_t* smp = sm.smp[buf];
       const double* czs = sm.cz[buf];
 
@@ -543,8 +558,8 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
       rbv += runv;
       rbf += tot >> 16;
       lds_barrier();
-      if (NBUF == 1 && zc + ZC < ze) {
-        fetch(zc + ZC, 0);
+      if (NBUF == 1 && znx < ze) {
+        fetch(znx, 0);
         __builtin_amdgcn_s_waitcnt(0);
         lds_barrier();
       }

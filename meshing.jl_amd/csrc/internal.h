@@ -48,6 +48,8 @@ struct Plan {
 
 struct Workspace {
   uint32_t* cnt;      // per unit: vertices | faces << 16
+  uint64_t* zmask;    // per unit: bit k set when z block k of the segment may
+                      // hold an active cell (zmask_shift)
   int64_t* voff;      // per unit: exclusive prefix of vertices
   int64_t* foff;      // per unit: exclusive prefix of faces
   uint64_t* tilesum;  // per scan tile: vertices | faces << 32
@@ -69,6 +71,7 @@ struct VolArgs {
   double ca[3], cb[3];     // LinRange ends per axis
   double* coords;          // xp | yp | zp of this volume
   uint32_t* cnt;           // unit counts, indexed from ubase
+  uint64_t* zmask;         // unit z-block activity masks, indexed from ubase
   const int64_t* voff;     // exclusive unit prefixes, indexed from ubase
   const int64_t* foff;
   const int64_t* sz;       // (first vertex, nverts, first face, nfaces) of this
@@ -93,6 +96,14 @@ inline int size_class(const msh_field* f) {
   const double cells = (double)(f->nx > 1 ? f->nx - 1 : 0) * (double)(f->ny > 1 ? f->ny - 1 : 0) *
                        (double)(f->nz > 1 ? f->nz - 1 : 0);
   return cells < 1048576.0 ? 0 : cells < 8388608.0 ? 1 : 2;
+}
+
+// z blocks of the activity masks: 16 cells for segments up to 1024 cells,
+// 32 up to 2048, 64 up to 4096 (so a mask has at most 64 blocks, and every
+// emit chunk of 4, 8 or 16 cells lies inside one block); returns log2 of the
+// block length.
+__host__ __device__ inline int zmask_shift(int zseg) {
+  return zseg <= 1024 ? 4 : zseg <= 2048 ? 5 : 6;
 }
 
 constexpr int kScanItems = 8;

@@ -833,6 +833,7 @@ VolArgs make_vol(const Plan& p, const Geom& g, const void* field, const Workspac
   }
   a.coords = w.coords;
   a.cnt = w.cnt;
+  a.zmask = w.zmask;
   a.voff = w.voff;
   a.foff = w.foff;
   a.sz = w.totals;

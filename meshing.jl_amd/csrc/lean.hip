@@ -133,6 +133,15 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
   uint32_t acc[TY];
 #pragma unroll
   for (int r = 0; r < TY; ++r) acc[r] = 0;
+  // z blocks where this lane's column of TY cells may hold an active cell:
+  // a cell is active when its 8 corner bits differ, and the column's cells
+  // overlap (row r spans sample rows r, r+1), so some cell of the column is
+  // active exactly when the column's bits in the two planes are not all
+  // equal (rows past the volume repeat its last row, which can only add
+  // blocks, never drop one)
+  constexpr uint32_t kFull = (1u << (2 * (TY + 1))) - 1;
+  const int zsh = zmask_shift(g.zseg);
+  uint64_t zm = 0;
 
   auto load_group = [&](int k, T (&v)[ZU][TY + 1]) {
 #pragma unroll
@@ -150,6 +159,9 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
       if (k + u >= ze) break;
       const uint32_t w1 = plane_bits(v[u]);
       const int kc = k + u;
+      const uint32_t b0 = w0 & kFull, b1 = w1 & kFull;
+      const bool act = b0 != b1 || (b0 != 0 && b0 != kFull);
+      zm |= (uint64_t)act << ((kc - zs) >> zsh);
       const uint32_t clsz = METHOD == MT ? ((kc == 0) ? 4u : 0u) : ((kc >= 1) ? 4u : 0u);
 #pragma unroll
       for (int r = 0; r < TY; ++r) {
@@ -184,10 +196,15 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
     k += ZU;
   }
   if (!xvalid) return;
+  uint64_t* __restrict__ zmask = a.zmask + a.ubase;
 #pragma unroll
   for (int r = 0; r < TY; ++r) {
     const int j = yb + r;
-    if (j < g.ny - 1) cnt[((int64_t)(xb + lane) * (g.ny - 1) + j) * g.nzseg + s] = acc[r];
+    if (j < g.ny - 1) {
+      const int64_t u = ((int64_t)(xb + lane) * (g.ny - 1) + j) * g.nzseg + s;
+      cnt[u] = acc[r];
+      zmask[u] = zm;
+    }
   }
 }
 

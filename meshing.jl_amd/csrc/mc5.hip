@@ -50,6 +50,7 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
     uint32_t list[kWaves][CAPE];         // x | z << 6 | case << 10 | first vertex slot << 18
     uint16_t lpf[kWaves][CAPE];          // first face slot
     uint64_t vbits[kWaves][CAPV / 64], fbits[kWaves][CAPF / 64];  // slot o starts an entry
+    uint64_t tzm[kWaves];                // the tile's z-block mask, per wave
   };
   __shared__ Smem sm;
   const int lane = lane_id(), wv = wave_id();
@@ -101,16 +102,20 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
     // this lane's row (ib + lane, jb + wv): output offsets, carried chunk to chunk
     const bool rowok = lane < TX && ib + lane < g.nx - 1 && jb + wv < g.ny - 1;
     int64_t rbv = 0, rbf = 0;
-    bool emits = false;
+    uint64_t zm = 0;
     if (rowok) {
       const int64_t u = a.ubase + ((int64_t)(ib + lane) * (g.ny - 1) + jb + wv) * g.nzseg + s;
       rbv = a.voff[u] - vbase;
       rbf = a.foff[u] - fbase;
-      emits = a.cnt[u] != 0;
+      zm = a.cnt[u] != 0 ? a.zmask[u] : 0ull;
     }
-    // a tile whose rows the count found empty writes nothing: skip it
-    // before fetching any of its bricks (sparse fields have many)
-    if (!__syncthreads_or(emits)) return;
+    // the z blocks where some row of the tile may hold an active cell: the
+    // chunks of the other blocks write nothing and are never fetched, and a
+    // tile with none is skipped whole (sparse fields have many of both)
+    const uint64_t tmask = tile_zmask(zm, sm.tzm);
+    if (tmask == 0) return;
+    const int zsh = zmask_shift(g.zseg);
+    const int z0 = zs + (__builtin_ctzll(tmask) << zsh);
 
     // DMA layout (as in the emit kernel with lanes along z): per-lane byte
     // offsets relative to the chunk's first sample are fixed.
@@ -167,14 +172,15 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
     for (int q = lane; q < CAPV / 64; q += kWave) vbits[q] = 0;
     for (int q = lane; q < CAPF / 64; q += kWave) fbits[q] = 0;
 
-    if (zs < ze) fetch(zs, 0);
+    if (z0 < ze) fetch(z0, 0);
     __builtin_amdgcn_s_waitcnt(0);
     lds_barrier();
     int buf = 0;
-    for (int zc = zs; zc < ze; zc += ZC, buf ^= NBUF - 1) {
+    for (int zc = z0, znx; zc < ze; zc = znx, buf ^= NBUF - 1) {
+      znx = next_zchunk(zc, ZC, zs, ze, zsh, tmask);
       // double-buffered: the next brick streams in during this chunk;
       // single-buffered: it is fetched after the chunk (more blocks per CU)
-      if (NBUF == 2 && zc + ZC < ze) fetch(zc + ZC, buf ^ 1);
+      if (NBUF == 2 && znx < ze) fetch(znx, buf ^ 1);
       const uint32_t* smp = sm.smp[buf];
       const double* czs = sm.cz[buf];
 
@@ -326,8 +332,8 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
       rbv += tot & 0xFFFF;
       rbf += tot >> 16;
       lds_barrier();
-      if (NBUF == 1 && zc + ZC < ze) {
-        fetch(zc + ZC, 0);
+      if (NBUF == 1 && znx < ze) {
+        fetch(znx, 0);
         __builtin_amdgcn_s_waitcnt(0);
         lds_barrier();
       }
