@@ -56,18 +56,29 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// The OR over the block of every thread's 64-bit value, with one barrier
-// (`slot`: kWaves words of LDS, free until every wave has passed the
-// barrier that starts the next tile): a tile's z-block activity mask from
-// its rows' masks.
-__device__ __forceinline__ uint64_t tile_zmask(uint64_t m, uint64_t* slot) {
+// The OR over the block of every thread's 64-bit value and of its flag,
+// with one barrier (`slot`: 2 * kWaves words of LDS, free until every wave
+// has passed the barrier that starts the next tile): a tile's z-block
+// activity mask from its rows' masks, and whether any row writes.
+__device__ __forceinline__ uint64_t tile_zmask(uint64_t m, bool flag, uint64_t* slot,
+                                               bool* any) {
   for (int d = kWave / 2; d >= 1; d >>= 1) m |= __shfl_xor(m, d, kWave);
-  if ((threadIdx.x & (kWave - 1)) == 0) slot[threadIdx.x / kWave] = m;
+  const bool f = __ballot(flag) != 0;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    slot[threadIdx.x / kWave] = m;
+    slot[kWaves + threadIdx.x / kWave] = f ? 1u : 0u;
+  }
   lds_barrier();
-  uint64_t t = 0;
+  uint64_t t = 0, fl = 0;
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) t |= slot[w];
-  return t;
+  for (int w = 0; w < kWaves; ++w) {
+    t |= slot[w];
+    fl |= slot[kWaves + w];
+  }
+  // the same in every lane: keep it (and the chunk walk it steers) scalar
+  *any = __builtin_amdgcn_readfirstlane((int)fl) != 0;
+  return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(t >> 32)) << 32) |
+         (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
 }
 
 // The first chunk after the one at zc (chunks of zcl cells from zs, which

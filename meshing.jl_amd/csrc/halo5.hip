@@ -64,7 +64,7 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
     // queue entries: x | z << 6 | case << 10 | class << 18 | first slot << 21
     uint32_t vlist[kWaves][CAPE], flist[kWaves][CAPE];
     uint64_t vbits[kWaves][CAPV / 64], fbits[kWaves][CAPF / 64];
-    uint64_t tzm[kWaves];  // the tile's z-block mask, per wave
+    uint64_t tzm[2 * kWaves];  // the tile's z-block mask and write flag, per wave
   };
   __shared__ Smem sm;
   const int lane = lane_id(), wv = wave_id();
@@ -162,25 +162,28 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
     uint64_t zm = 0;
     if (exists) {
       const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + gj) * g.nzseg + s;
+      const uint64_t zu = zmask_at<T>(a, gi, gj, s);  // loaded beside the offsets
+      const uint32_t cu = a.cnt[u];
       rbv = a.voff[u] - vbase;
       rbf = a.foff[u] - fbase;
-      writes = emits && a.cnt[u] != 0;
-      zm = a.cnt[u] != 0 ? a.zmask[u] : 0ull;
+      writes = emits && cu != 0;
+      zm = cu != 0 ? zu : 0ull;
+    }
+    if (hexists) {
+      const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + y0) * g.nzseg + s;
+      const uint64_t zu = zmask_at<T>(a, gi, y0, s);
+      hrbv = a.voff[u] - vbase;
+      zm |= a.cnt[u] != 0 ? zu : 0ull;
     }
     // a tile whose emitting rows the count found empty writes nothing (the
     // rows around it only supply owners to faces): skip it before fetching
-    // any of its bricks
-    if (!__syncthreads_or(writes)) return;
-    if (hexists) {
-      const int64_t u = a.ubase + ((int64_t)gi * (g.ny - 1) + y0) * g.nzseg + s;
-      hrbv = a.voff[u] - vbase;
-      zm |= a.cnt[u] != 0 ? a.zmask[u] : 0ull;
-    }
-    // the z blocks where some row of the brick, the rows around it included,
-    // may hold an active cell: the chunks of the other blocks change no
-    // row's vertex offset, write nothing and are never fetched
-    const uint64_t tmask = tile_zmask(zm, sm.tzm);
-    if (tmask == 0) return;  // (an emitting row with outputs marks a block)
+    // any of its bricks.  Otherwise only the z blocks where some row of the
+    // brick, the rows around it included, may hold an active cell are
+    // walked: the chunks of the other blocks change no row's vertex offset,
+    // write nothing and are never fetched.
+    bool any;
+    const uint64_t tmask = tile_zmask(zm, writes, sm.tzm, &any);
+    if (!any || tmask == 0) return;  // (a writing row marks a block)
     const int zsh = zmask_shift(g.zseg);
     const int z0 = zs + (__builtin_ctzll(tmask) << zsh);
 

@@ -48,8 +48,9 @@ struct Plan {
 
 struct Workspace {
   uint32_t* cnt;      // per unit: vertices | faces << 16
-  uint64_t* zmask;    // per unit: bit k set when z block k of the segment may
-                      // hold an active cell (zmask_shift)
+  uint64_t* zmask;    // per count row group (zmask_at): bit k set when z
+                      // block k of the segment holds a cell with outputs
+                      // (zmask_shift); room for one per unit
   int64_t* voff;      // per unit: exclusive prefix of vertices
   int64_t* foff;      // per unit: exclusive prefix of faces
   uint64_t* tilesum;  // per scan tile: vertices | faces << 32
@@ -57,6 +58,10 @@ struct Workspace {
   int64_t* totals;    // (0, nverts, 0, nfaces): the sizes of a single volume
   double* coords;     // xp[nx], yp[ny], zp[nz] as double
 };
+
+// cell rows along y per count tile (one lane marches TY rows)
+template <typename T>
+__host__ __device__ constexpr int count_ty() { return sizeof(T) == 4 ? 8 : 6; }
 
 // Everything the kernels need to know about one volume.  Single-volume
 // launches take it by value; batched launches read one entry per volume
@@ -71,7 +76,7 @@ struct VolArgs {
   double ca[3], cb[3];     // LinRange ends per axis
   double* coords;          // xp | yp | zp of this volume
   uint32_t* cnt;           // unit counts, indexed from ubase
-  uint64_t* zmask;         // unit z-block activity masks, indexed from ubase
+  uint64_t* zmask;         // z-block activity masks per count row group, from ubase
   const int64_t* voff;     // exclusive unit prefixes, indexed from ubase
   const int64_t* foff;
   const int64_t* sz;       // (first vertex, nverts, first face, nfaces) of this
@@ -87,6 +92,14 @@ struct VolArgs {
   int64_t ntiles;          // emit tiles
   int64_t eblk0;           // first emit block of this volume in a batched launch
 };
+
+// The z-block mask of cell row (gi, gj) in z segment s: the count keeps one
+// per lane and count tile, i.e. per count_ty<T>() consecutive rows along y,
+// so a row's mask may mark blocks of its neighbours too.
+template <typename T>
+__host__ __device__ inline uint64_t zmask_at(const VolArgs& a, int gi, int gj, int s) {
+  return a.zmask[a.ubase + ((int64_t)gi * a.cnby + gj / count_ty<T>()) * a.g.nzseg + s];
+}
 
 // Size classes of the kernel-family choice (msh_autotune keeps one choice
 // per method, element type and class): volumes of fewer than 2^20 cells

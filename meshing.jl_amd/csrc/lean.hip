@@ -114,14 +114,14 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
 #pragma unroll
   for (int r = 0; r <= TY; ++r)
     voff[r] = ((uint32_t)min(yb + r, g.ny - 1) * (uint32_t)g.nx + (uint32_t)xs) * sizeof(T);
+  // class bits of the per-cell table (MT, SN): x from the lane, y per row
+  // (the same in every lane, so they stay in scalar registers), z per plane
   uint32_t clsx = 0, clsy[TY];
   if (METHOD == MT) clsx = (xb + lane == 0) ? 1u : 0u;
   if (METHOD == SN) clsx = (xb + lane >= 1) ? 1u : 0u;
 #pragma unroll
-  for (int r = 0; r < TY; ++r) {
+  for (int r = 0; r < TY; ++r)
     clsy[r] = METHOD == MT ? ((yb + r == 0) ? 2u : 0u) : ((yb + r >= 1) ? 2u : 0u);
-    clsy[r] |= clsx;
-  }
 
   auto plane_bits = [&](const T* v) {
     uint32_t w = 0;
@@ -133,15 +133,12 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
   uint32_t acc[TY];
 #pragma unroll
   for (int r = 0; r < TY; ++r) acc[r] = 0;
-  // z blocks where this lane's column of TY cells may hold an active cell:
-  // a cell is active when its 8 corner bits differ, and the column's cells
-  // overlap (row r spans sample rows r, r+1), so some cell of the column is
-  // active exactly when the column's bits in the two planes are not all
-  // equal (rows past the volume repeat its last row, which can only add
-  // blocks, never drop one)
-  constexpr uint32_t kFull = (1u << (2 * (TY + 1))) - 1;
+  // z blocks where some cell of this lane's TY rows has outputs: a block is
+  // marked when the rows' running counts grew across it (the packed sum
+  // of a block's growth stays far below 2^32, so it cannot wrap to zero)
   const int zsh = zmask_shift(g.zseg);
   uint64_t zm = 0;
+  uint32_t tprev = 0;
 
   auto load_group = [&](int k, T (&v)[ZU][TY + 1]) {
 #pragma unroll
@@ -159,14 +156,11 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
       if (k + u >= ze) break;
       const uint32_t w1 = plane_bits(v[u]);
       const int kc = k + u;
-      const uint32_t b0 = w0 & kFull, b1 = w1 & kFull;
-      const bool act = b0 != b1 || (b0 != 0 && b0 != kFull);
-      zm |= (uint64_t)act << ((kc - zs) >> zsh);
-      const uint32_t clsz = METHOD == MT ? ((kc == 0) ? 4u : 0u) : ((kc >= 1) ? 4u : 0u);
+      const uint32_t clxz = clsx | (METHOD == MT ? ((kc == 0) ? 4u : 0u) : ((kc >= 1) ? 4u : 0u));
 #pragma unroll
       for (int r = 0; r < TY; ++r) {
         const uint32_t cs = ((w0 >> (2 * r)) & 0xF) | (((w1 >> (2 * r)) & 0xF) << 4);
-        const uint32_t idx = METHOD == MC ? cs : (cs | ((clsy[r] | clsz) << 8));
+        const uint32_t idx = METHOD == MC ? cs : (cs | ((clsy[r] | clxz) << 8));
         acc[r] += lut[idx];
       }
       w0 = w1;
@@ -179,6 +173,17 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
     for (int r = 0; r <= TY; ++r) v0[r] = buffer_load<T>(rs, voff[r]);
     w0 = plane_bits(v0);
   }
+  // after the group ending before cell k: close the z block when it ends
+  // there (blocks are whole groups; the last one may be short)
+  auto close_block = [&](int k) {
+    if (((k - zs) & ((1 << zsh) - 1)) == 0 || k >= ze) {
+      uint32_t t = 0;
+#pragma unroll
+      for (int r = 0; r < TY; ++r) t += acc[r];
+      zm |= (uint64_t)(t != tprev) << ((k - 1 - zs) >> zsh);
+      tprev = t;
+    }
+  };
   // two register sets: the next group's loads are in flight while the
   // current group is classified
   T va[ZU][TY + 1], vb[ZU][TY + 1];
@@ -189,23 +194,22 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
     __builtin_amdgcn_sched_barrier(0);
     use_group(k, va);
     k += ZU;
+    close_block(k);
     if (k >= ze) break;
     if (k + ZU < ze) load_group(k + ZU, va);
     __builtin_amdgcn_sched_barrier(0);
     use_group(k, vb);
     k += ZU;
+    close_block(k);
   }
   if (!xvalid) return;
-  uint64_t* __restrict__ zmask = a.zmask + a.ubase;
 #pragma unroll
   for (int r = 0; r < TY; ++r) {
     const int j = yb + r;
-    if (j < g.ny - 1) {
-      const int64_t u = ((int64_t)(xb + lane) * (g.ny - 1) + j) * g.nzseg + s;
-      cnt[u] = acc[r];
-      zmask[u] = zm;
-    }
+    if (j < g.ny - 1) cnt[((int64_t)(xb + lane) * (g.ny - 1) + j) * g.nzseg + s] = acc[r];
   }
+  // one mask for the lane's TY rows (zmask_at)
+  a.zmask[a.ubase + ((int64_t)(xb + lane) * nby + by) * g.nzseg + s] = zm;
 }
 
 template <typename T, int METHOD, int TY, int ZU>
@@ -553,8 +557,6 @@ __global__ __launch_bounds__(kBlock) void k_emit_mc4_b(const VolArgs* __restrict
 // ---------------------------------------------------------------------------
 // Host side.
 
-template <typename T>
-static constexpr int count_ty() { return sizeof(T) == 4 ? 8 : 6; }
 // planes per load group (two groups in flight): 2 for Float32, 4 for
 // Float64 measured best at 1024^3 (Float32 1.14 vs 1.18 ms with 4)
 template <typename T>

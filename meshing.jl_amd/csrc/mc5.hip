@@ -50,7 +50,7 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
     uint32_t list[kWaves][CAPE];         // x | z << 6 | case << 10 | first vertex slot << 18
     uint16_t lpf[kWaves][CAPE];          // first face slot
     uint64_t vbits[kWaves][CAPV / 64], fbits[kWaves][CAPF / 64];  // slot o starts an entry
-    uint64_t tzm[kWaves];                // the tile's z-block mask, per wave
+    uint64_t tzm[2 * kWaves];            // the tile's z-block mask and write flag, per wave
   };
   __shared__ Smem sm;
   const int lane = lane_id(), wv = wave_id();
@@ -105,14 +105,16 @@ __device__ __forceinline__ void emit_mc5_run(Src src) {
     uint64_t zm = 0;
     if (rowok) {
       const int64_t u = a.ubase + ((int64_t)(ib + lane) * (g.ny - 1) + jb + wv) * g.nzseg + s;
+      const uint64_t zu = zmask_at<T>(a, ib + lane, jb + wv, s);  // beside the offsets
       rbv = a.voff[u] - vbase;
       rbf = a.foff[u] - fbase;
-      zm = a.cnt[u] != 0 ? a.zmask[u] : 0ull;
+      zm = a.cnt[u] != 0 ? zu : 0ull;
     }
     // the z blocks where some row of the tile may hold an active cell: the
     // chunks of the other blocks write nothing and are never fetched, and a
     // tile with none is skipped whole (sparse fields have many of both)
-    const uint64_t tmask = tile_zmask(zm, sm.tzm);
+    bool any;
+    const uint64_t tmask = tile_zmask(zm, zm != 0, sm.tzm, &any);
     if (tmask == 0) return;
     const int zsh = zmask_shift(g.zseg);
     const int z0 = zs + (__builtin_ctzll(tmask) << zsh);
