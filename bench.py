@@ -33,6 +33,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -548,47 +549,46 @@ def summarize_counters(fetch_dir, write_dir, line_file, out_file=TRAFFIC_FILE, c
            "scale": {k: round(v, 4) for k, v in scale.items()}, "workloads": {}}
     per = [{} for _ in loads]
     for counter, rs in (("FETCH_SIZE", fetch), ("WRITE_SIZE", write)):
-        groups = {}  # (kernel, grid) -> workloads resolving to it, in bench order
-        found = {}
+        # each workload starts by generating its field, so workload i (in
+        # bench order) owns the dispatches from the i-th field generation up
+        # to the next: other workloads' autotuning launches the same
+        # instantiations at the same grids
+        gens = sorted(int(x["Dispatch_Id"]) for x in rs if "k_generate" in x["Kernel_Name"])
         for i, (key, meth, dtype, r, pick) in enumerate(loads):
-            pat = _emit_pattern(meth, dtype, r["family_id"])
-            grids = {}
-            for x in rs:
-                if pat.search(x["Kernel_Name"]):
-                    grids.setdefault((x["Kernel_Name"], int(x["Grid_Size"])), []).append(
-                        (int(x["Dispatch_Id"]), float(x["Counter_Value"])))
-            if not grids:
+            lo = gens[i] if i < len(gens) else 0
+            hi = gens[i + 1] if i + 1 < len(gens) else float("inf")
+            t = "float" if dtype == "float32" else "double"
+            mid = {"mc": 0, "mt": 1, "sn": 2}[meth]
+            for part, pat in (("emit", _emit_pattern(meth, dtype, r["family_id"])),
+                              ("count", re.compile(rf"k_count4<{t}, {mid},"))):
+                grids = {}
+                for x in rs:
+                    if lo <= int(x["Dispatch_Id"]) < hi and pat.search(x["Kernel_Name"]):
+                        grids.setdefault((x["Kernel_Name"], int(x["Grid_Size"])), []).append(
+                            float(x["Counter_Value"]))
+                if not grids:
+                    continue
+                # the 1024^3 launches have the largest grid of their
+                # instantiation, the 256^3 ones the smallest
+                g = pick(grids, key=lambda k: k[1])
+                vals = grids[g]
+                per[i].setdefault(part, {})[counter] = dict(
+                    kernel=g[0], grid=g[1], dispatches=len(vals), kib=sum(vals) / len(vals))
+    for (key, meth, dtype, r, pick), pw in zip(loads, per):
+        for part, pc in pw.items():
+            if len(pc) < 2:
                 continue
-            # the 1024^3 launches have the largest grid of their instantiation,
-            # the 256^3 ones the smallest
-            g = pick(grids, key=lambda k: k[1])
-            found[g] = sorted(grids[g])
-            groups.setdefault(g, []).append(i)
-        for g, idxs in groups.items():
-            ds = found[g]
-            # workloads on the same instantiation and grid (Float32 value noise
-            # and the sphere at 1024^3) ran one after the other: split the
-            # dispatches at the widest gaps in dispatch id, many other
-            # workloads' dispatches lie between them
-            cuts = sorted(sorted(range(1, len(ds)), key=lambda j: ds[j][0] - ds[j - 1][0],
-                                 reverse=True)[:len(idxs) - 1])
-            parts = [ds[lo:hi] for lo, hi in zip([0] + cuts, cuts + [len(ds)])]
-            for i, part in zip(idxs, parts):
-                if part:
-                    per[i][counter] = dict(kernel=g[0], grid=g[1], dispatches=len(part),
-                                           kib=sum(v for _, v in part) / len(part))
-    for (key, meth, dtype, r, pick), pc in zip(loads, per):
-        if len(pc) < 2:
-            continue
-        fam = r["family_id"]
-        fb = pc["FETCH_SIZE"]["kib"] * 1024 * scale["FETCH_SIZE"]
-        wb = pc["WRITE_SIZE"]["kib"] * 1024 * scale["WRITE_SIZE"]
-        alg = r.get("algorithmic_bytes") or r["roofline"]["algorithmic_bytes_per_launch"]
-        out["workloads"][key] = dict(
-            kernel=pc["FETCH_SIZE"]["kernel"], grid=pc["FETCH_SIZE"]["grid"],
-            family_id=fam, dispatches=pc["FETCH_SIZE"]["dispatches"],
-            fetch_bytes=int(fb), write_bytes=int(wb), bytes_per_launch=int(fb + wb),
-            algorithmic_bytes=alg, traffic_over_algorithmic=round((fb + wb) / alg, 3))
+            fam = r["family_id"]
+            fb = pc["FETCH_SIZE"]["kib"] * 1024 * scale["FETCH_SIZE"]
+            wb = pc["WRITE_SIZE"]["kib"] * 1024 * scale["WRITE_SIZE"]
+            # emit: field + outputs; count: the field (one read of every sample)
+            alg = r.get("algorithmic_bytes") if part == "emit" else r.get("field_bytes")
+            e = dict(kernel=pc["FETCH_SIZE"]["kernel"], grid=pc["FETCH_SIZE"]["grid"],
+                     family_id=fam, dispatches=pc["FETCH_SIZE"]["dispatches"],
+                     fetch_bytes=int(fb), write_bytes=int(wb), bytes_per_launch=int(fb + wb))
+            if alg:
+                e.update(algorithmic_bytes=alg, traffic_over_algorithmic=round((fb + wb) / alg, 3))
+            out["workloads"][key if part == "emit" else key + "_count"] = e
     Path(out_file).write_text(json.dumps(out, indent=1) + "\n")
     return out
 
@@ -643,17 +643,25 @@ def main():
     m = entry.load_package()
     torch.cuda.init()
     def roofline(r, meth, workload):
-        b = r["field_bytes"] + r["out_bytes"]
-        ach = b / r["emit_s"]
-        traffic = traffic_of(workload, r["family_id"])
+        if r["count_s"] > r["emit_s"]:
+            # a sparse field: the emit fetches only the z chunks with active
+            # cells, and the count pass, which reads every sample once,
+            # dominates (its time includes the scan and the size read-back)
+            b, t, kernel = r["field_bytes"], r["count_s"], "k_count4"
+            traffic = traffic_of(workload + "_count", r["family_id"])
+        else:
+            b, t = r["field_bytes"] + r["out_bytes"], r["emit_s"]
+            kernel = _EMIT_NAMES[meth].get(r["family_id"], "k_emit")
+            traffic = traffic_of(workload, r["family_id"])
+        ach = b / t
         out = {"bound": "hbm", "achieved": round(ach / 1e9, 2), "peak": HBM_PEAK / 1e9,
                "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4), "traffic": traffic,
-               "kernel": _EMIT_NAMES[meth].get(r["family_id"], "k_emit"),
-               "algorithmic_bytes_per_launch": b, "avg_launch_ms": round(r["emit_s"] * 1e3, 4)}
+               "kernel": kernel, "algorithmic_bytes_per_launch": b,
+               "avg_launch_ms": round(t * 1e3, 4)}
         if traffic:
             # the bytes the counters saw the kernel move, over the same launch time
-            out["counter_GBps"] = round(traffic / r["emit_s"] / 1e9, 1)
-            out["counter_frac"] = round(traffic / r["emit_s"] / HBM_PEAK, 4)
+            out["counter_GBps"] = round(traffic / t / 1e9, 1)
+            out["counter_frac"] = round(traffic / t / HBM_PEAK, 4)
         return out
 
     f64 = measure(m, "float64", a.n, a.method, a.steps, a.warmup, with_weld=True)
@@ -667,7 +675,8 @@ def main():
                             ms_per_step=round(r["step_s"] * 1e3, 4),
                             count_ms=round(r["count_s"] * 1e3, 4),
                             emit_ms=round(r["emit_s"] * 1e3, 4),
-                            nverts=r["nverts"], nfaces=r["nfaces"],
+                            nverts=r["nverts"], nfaces=r["nfaces"], field_bytes=r["field_bytes"],
+                            algorithmic_bytes=r["field_bytes"] + r["out_bytes"],
                             roofline=roofline(r, meth, f"{meth}_gyroid_256_float32"))
     # the sparse 1024^3 point: Float32 sphere of radius 0.5 (few outputs, so
     # the reads dominate)
@@ -676,7 +685,8 @@ def main():
                   autotuned_emit_kernels=sph["family"], family_id=sph["family_id"],
                   ms_per_step=round(sph["step_s"] * 1e3, 4),
                   count_ms=round(sph["count_s"] * 1e3, 4), emit_ms=round(sph["emit_s"] * 1e3, 4),
-                  nverts=sph["nverts"], nfaces=sph["nfaces"],
+                  nverts=sph["nverts"], nfaces=sph["nfaces"], field_bytes=sph["field_bytes"],
+                  algorithmic_bytes=sph["field_bytes"] + sph["out_bytes"],
                   step_hbm_frac=round((sph["field_bytes"] + sph["out_bytes"]) / sph["step_s"]
                                       / HBM_PEAK, 4),
                   roofline=roofline(sph, "mc", f"mc_sphere_{a.n}_float32"))
@@ -695,6 +705,7 @@ def main():
                     ms_per_step=round(r["step_s"] * 1e3, 4),
                     count_ms=round(r["count_s"] * 1e3, 4), emit_ms=round(r["emit_s"] * 1e3, 4),
                     nverts=r["nverts"], nfaces=r["nfaces"], algorithmic_bytes=b,
+                    field_bytes=r["field_bytes"],
                     autotuned_emit_kernels=r["family"], family_id=r["family_id"],
                     step_hbm_frac=round(b / r["step_s"] / HBM_PEAK, 4),
                     roofline=roofline(r, a.method, f"{a.method}_value_noise_{a.n}_{r['dtype']}"),

@@ -36,9 +36,9 @@ def test_summary_selects_timed_instantiation(tmp_path):
     _csv(tmp_path / "w", "WRITE_SIZE", write)
     line = {"float64": {"family_id": 2, "algorithmic_bytes": 1000},
             "float32": {"family_id": 2, "algorithmic_bytes": 1000},
-            "gyroid_256_f32": {"mt": {"family_id": 2, "roofline": {"algorithmic_bytes_per_launch": 10}},
-                               "sn": {"family_id": 2, "roofline": {"algorithmic_bytes_per_launch": 10}},
-                               "mc": {"family_id": 2, "roofline": {"algorithmic_bytes_per_launch": 10}}}}
+            "gyroid_256_f32": {"mt": {"family_id": 2, "algorithmic_bytes": 10},
+                               "sn": {"family_id": 2, "algorithmic_bytes": 10},
+                               "mc": {"family_id": 2, "algorithmic_bytes": 10}}}
     (tmp_path / "line.json").write_text("noise\n" + json.dumps(line) + "\n")
     out = bench.summarize_counters(tmp_path / "f", tmp_path / "w", tmp_path / "line.json",
                                    out_file=tmp_path / "t.json")
@@ -58,23 +58,26 @@ def test_summary_selects_timed_instantiation(tmp_path):
 
 def test_summary_splits_workloads_on_one_instantiation(tmp_path):
     # Float32 value noise and the Float32 sphere at 1024^3 run the same
-    # instantiation at the same grid, one after the other with the gyroid
-    # workloads between them: each takes its own run of dispatches
+    # instantiations at the same grid (each autotunes over the other's
+    # family too): each workload takes the dispatches from its own field
+    # generation to the next
     f32 = "msh::(anonymous namespace)::k_emit_mc5<float, float, float, 16, true, 0, 2>(msh::VolArgs)"
+    f32h = "msh::(anonymous namespace)::k_emit_mc5<float, float, float, 8, true, 0, 1>(msh::VolArgs)"
     f64 = "msh::(anonymous namespace)::k_emit_mc5<double, double, double, 8, true, 0, 2>(msh::VolArgs)"
-    other = "msh::(anonymous namespace)::k_generate<float>(long)"
+    gen = "msh::(anonymous namespace)::k_generate<float>(long)"
     copy = "__amd_rocclr_copyBuffer"
 
     def rows(vn, sph, kib_of_copy):
-        r = [(f64, 1000, 1.0)] + [(f32, 900, vn)] * 3 + [(other, 7, 0.0)] * 40 + \
-            [(f32, 900, sph)] * 2
-        return r + [(copy, 1, kib_of_copy)] * 4
+        r = [(gen, 7, 0.0), (f64, 1000, 1.0)]
+        r += [(gen, 7, 0.0)] + [(f32h, 900, 99.0)] + [(f32, 900, vn)] * 3
+        r += [(gen, 7, 0.0)] + [(f32, 900, 77.0)] + [(f32h, 900, sph)] * 2
+        return r + [(gen, 7, 0.0)] + [(copy, 1, kib_of_copy)] * 4
 
     _csv(tmp_path / "f", "FETCH_SIZE", rows(40.0, 20.0, 524288.0))
     _csv(tmp_path / "w", "WRITE_SIZE", rows(30.0, 1.0, 1048576.0))
     line = {"float64": {"family_id": 2, "algorithmic_bytes": 1000},
             "float32": {"family_id": 2, "algorithmic_bytes": 1000},
-            "sphere_1024_f32": {"family_id": 2, "roofline": {"algorithmic_bytes_per_launch": 500}}}
+            "sphere_1024_f32": {"family_id": 3, "algorithmic_bytes": 500}}
     (tmp_path / "line.json").write_text(json.dumps(line) + "\n")
     out = bench.summarize_counters(tmp_path / "f", tmp_path / "w", tmp_path / "line.json",
                                    out_file=tmp_path / "t.json")
