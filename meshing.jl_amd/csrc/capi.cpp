@@ -31,7 +31,28 @@ struct Launch {
   int variant = 0;          // emit variant (emit launches only)
   int64_t gbase = 0;        // first block in the block -> volume map
   int64_t nblocks = 0;
+  double cost = 0;          // estimated work: cells weighted by method
 };
+
+// Estimated work of a volume in a batched launch, for spreading the launches
+// over streams: its tile-cells (a partly filled tile costs about what a full
+// one does), and for an emit weighted by method (MC 1, surface nets 2,
+// tetrahedra 3: the 256^3 gyroid emits take 0.056, 0.12 and 0.17 ms);
+// counting costs the same per cell for every method.
+double cell_cost(const VolArgs& v, const Plan& p, bool emit) {
+  const double zc = (double)(v.g.zseg > 0 ? v.g.zseg : 0);
+  return emit ? (double)v.ntiles * zc * (p.method == MT ? 3.0 : p.method == SN ? 2.0 : 1.0)
+              : (double)v.cntiles * zc;
+}
+
+// Launch order for spreading: most work first (each goes to the stream with
+// the least work so far, so the short launches fill in at the end).
+std::vector<size_t> by_cost(const std::vector<Launch>& ls) {
+  std::vector<size_t> o(ls.size());
+  for (size_t i = 0; i < o.size(); ++i) o[i] = i;
+  std::stable_sort(o.begin(), o.end(), [&](size_t x, size_t y) { return ls[x].cost > ls[y].cost; });
+  return o;
+}
 
 // A captured phase of the batched form.  A batch whose arguments and
 // device buffers repeat those of the previous call runs its device work from
@@ -164,7 +185,7 @@ struct Spread {
   Ctx* c;
   int ways = 1;    // streams launches go to
   int forked = 0;  // aux streams that waited on the fork
-  std::vector<int64_t> load;
+  std::vector<double> load;
   hipError_t start(int nlaunch) {
     ways = nlaunch < kSpreadWays ? (nlaunch < 1 ? 1 : nlaunch) : kSpreadWays;
     load.assign((size_t)ways, 0);
@@ -184,8 +205,8 @@ struct Spread {
     if (e != hipSuccess) ways = 1 + forked;  // launch only on streams that forked
     return e;
   }
-  // the stream with the least work so far takes a launch of `work` tiles
-  hipStream_t next(int64_t work) {
+  // the stream with the least work so far takes a launch of `work`
+  hipStream_t next(double work) {
     int k = 0;
     for (int i = 1; i < ways; ++i)
       if (load[i] < load[k]) k = i;
@@ -777,6 +798,7 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
       done[u] = 1;
       b.vols[u].cblk0 = cblocks;
       cblocks += b.vols[u].cntiles;
+      L.cost += cell_cost(b.vols[u], b.plans[u], false);
     }
     L.nblocks = cblocks - L.gbase;
     if (L.nblocks) b.counts.push_back(L);
@@ -794,6 +816,7 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
       done[u] = 1;
       b.vols[u].eblk0 = eblocks;
       eblocks += b.vols[u].ntiles;
+      L.cost += cell_cost(b.vols[u], b.plans[u], true);
     }
     L.nblocks = eblocks - L.gbase;
     if (L.nblocks) b.emits.push_back(L);
@@ -848,9 +871,11 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
     if (e == hipSuccess) e = launch_zero_u32(ctrs, (int64_t)(8 * b.counts.size()), c->stream);
     Spread sp{c};
     if (e == hipSuccess) e = sp.start((int)b.counts.size());
-    for (size_t k = 0; k < b.counts.size() && e == hipSuccess; ++k)
+    for (size_t k : by_cost(b.counts)) {
+      if (e != hipSuccess) break;
       e = launch_count_batch(b.counts[k].plan, dvols, cmap, b.counts[k].gbase,
-                             b.counts[k].nblocks, ctrs + 8 * k, sp.next(b.counts[k].nblocks));
+                             b.counts[k].nblocks, ctrs + 8 * k, sp.next(b.counts[k].cost));
+    }
     const hipError_t ej = sp.finish();
     if (e == hipSuccess) e = ej;
     Geom gall{};
@@ -949,9 +974,11 @@ int msh_batch_extract(void* ctx, int32_t n, const msh_field* fields, const msh_p
     if (e == hipSuccess) e = launch_zero_u32(ectr, (int64_t)(8 * b.emits.size()), c->stream);
     Spread sp{c};
     if (e == hipSuccess) e = sp.start((int)b.emits.size());
-    for (size_t k = 0; k < b.emits.size() && e == hipSuccess; ++k)
+    for (size_t k : by_cost(b.emits)) {
+      if (e != hipSuccess) break;
       e = launch_emit_batch(b.emits[k].plan, b.emits[k].variant, dvols, emap, b.emits[k].gbase,
-                            b.emits[k].nblocks, ectr + 8 * k, sp.next(b.emits[k].nblocks));
+                            b.emits[k].nblocks, ectr + 8 * k, sp.next(b.emits[k].cost));
+    }
     const hipError_t ej = sp.finish();
     if (e == hipSuccess) e = ej;
     return e;
