@@ -1,8 +1,9 @@
 // noalloc.hip -- the device scratch of msh_isosurface_noalloc, the extraction
 // entry point that allocates nothing.
 //
-// The count/scan workspace of that entry point (per-unit counts and output
-// prefixes, the scan's look-back words and totals, the coordinate tables)
+// The count/scan workspace of that entry point (per-unit counts, z-block
+// masks and output prefixes, the scan's tile sums and offsets, the totals,
+// the coordinate tables)
 // lives in one file-scope device array of kScratchBytes.  Every GPU the
 // library runs on holds its own copy of the array, at its own address, for
 // the life of the process.  Host code never passes the array's name to a
