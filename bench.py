@@ -16,7 +16,9 @@ Marching Tetrahedra / surface nets / Marching Cubes workloads.
 
 Printed: one JSON line.  `value` is Gvoxels/s (samples per second / 1e9).
 `roofline` is for the dominant kernel (the Marching Cubes emit kernel,
-k_emit_mc5), from HIP events on the stream the library launches on; its
+k_emit_mc5; on the sparse sphere the count kernel, since the emit fetches
+only the z chunks with active cells), from HIP events on the stream the
+library launches on; its
 `traffic` is the HBM bytes per launch from the rocprofv3 FETCH_SIZE and
 WRITE_SIZE summaries kept in profiles/ (null when absent).  `cpu_baseline`
 times the CPU oracle (single thread) on an x-slab of the same field, and
@@ -642,8 +644,8 @@ def main():
     import __graft_entry__ as entry
     m = entry.load_package()
     torch.cuda.init()
-    def roofline(r, meth, workload):
-        if r["count_s"] > r["emit_s"]:
+    def roofline(r, meth, workload, count=False):
+        if count:
             # a sparse field: the emit fetches only the z chunks with active
             # cells, and the count pass, which reads every sample once,
             # dominates (its time includes the scan and the size read-back)
@@ -689,7 +691,7 @@ def main():
                   algorithmic_bytes=sph["field_bytes"] + sph["out_bytes"],
                   step_hbm_frac=round((sph["field_bytes"] + sph["out_bytes"]) / sph["step_s"]
                                       / HBM_PEAK, 4),
-                  roofline=roofline(sph, "mc", f"mc_sphere_{a.n}_float32"))
+                  roofline=roofline(sph, "mc", f"mc_sphere_{a.n}_float32", count=True))
     batch = measure_batch(m, a.batch_volumes, max(2, min(a.steps, 5))) if a.batch_volumes else None
     serve = (measure_serve(m, a.serve_requests, [float(q) for q in a.serve_qps.split(",")])
              if a.serve_requests else None)
