@@ -68,6 +68,33 @@ def test_family_skips_empty_tiles(pkg, oracle, reset_families, family, name, dty
     assert_same_mesh(res[2][0].cpu().numpy(), res[2][1].cpu().numpy(), Vr, Fr)
 
 
+def tall_field(nz, dtype):
+    # a thin column with a few separated blobs along z: in a batch its z
+    # rows are one segment of nz - 1 cells, so the z-block masks use blocks
+    # of 32 (segments up to 2048 cells) or 64 cells (up to 4096)
+    x, y, z = np.meshgrid(np.linspace(-1, 1, 24), np.linspace(-1, 1, 20),
+                          np.linspace(-1, 1, nz), indexing="ij")
+    f = np.full(x.shape, 1.0)
+    for zc in (-0.9, -0.2, 0.05, 0.7):
+        f = np.minimum(f, np.sqrt(x * x + y * y + ((z - zc) * 20) ** 2) - 0.5)
+    return f.astype(dtype)
+
+
+@pytest.mark.parametrize("family", [3, 2])
+@pytest.mark.parametrize("name,dtype", CASES)
+def test_family_long_z_segments(pkg, oracle, reset_families, family, name, dtype):
+    m = method_of(pkg, name)
+    pkg.set_kernel_family(m, dtype, family)
+    fields = [tall_field(1500, dtype), tall_field(3000, dtype)]
+    res = pkg.isosurface_batch([dev_field(f) for f in fields], m)
+    for f, (V, F) in zip(fields, res):
+        Vo, Fo = oracle.isosurface(f, name, iso=0)
+        assert len(Fo) > 100
+        assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
+        V1, F1 = pkg.isosurface(dev_field(f), m)  # single volume: shorter z segments
+        assert_same_mesh(V1.cpu().numpy(), F1.cpu().numpy(), Vo, Fo)
+
+
 @pytest.mark.parametrize("name,dtype", CASES)
 def test_autotune_keeps_results(pkg, oracle, reset_families, name, dtype):
     m = method_of(pkg, name)
