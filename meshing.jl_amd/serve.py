@@ -9,7 +9,9 @@ device, equal bit for bit to ``isosurface(field, method, X, Y, Z)`` called
 alone.  The results were allocated on a server stream and are complete when
 the future resolves; a client that hands them to asynchronous work on
 another stream and then drops them should call ``record_stream`` first, as
-for any tensor shared across streams.
+for any tensor shared across streams.  The vertices and faces of a result
+are views of one allocation (half the allocator calls on the scheduler's
+critical path), freed when both are dropped.
 
 The scheduler thread forms a batch from every queued request that fits the
 HBM budget (fields + workspace + an output estimate), runs
@@ -26,7 +28,9 @@ and moves on: the next batch's classify/count runs on the other stream
 while this batch's emit is still on the GPU.  A completion thread waits for
 each batch's event, resolves its futures and returns its memory to the
 budget, which lets the scheduler admit the requests that arrived meanwhile.
-Streams keep the default priority.
+The two streams are the device's, shared by every Server the process
+opens on it (servers open at the same time queue behind each other's
+work), and keep the default priority.
 
 Input contract: a resident field is read on a server stream after the
 work that the client's current stream has queued when ``submit`` is called
@@ -49,6 +53,36 @@ from . import _capi
 from ._capi import MSH_DEVICE, MSH_FLOAT32, MshField, MshParams
 
 __all__ = ["FieldSpec", "Server"]
+
+
+_streams = {}
+_streams_lock = threading.Lock()
+
+
+def _server_streams(torch, device):
+    """The two streams every Server on ``device`` runs on.  torch caches
+    freed memory per stream and serves an allocation only from its own
+    stream's cache, so servers that a process opens one after another
+    reuse what the earlier ones cached rather than each growing a cache of
+    its own until the device runs out (torch then frees every cache at once
+    and maps the memory anew: seconds, at the sizes served here)."""
+    with _streams_lock:
+        if device.index not in _streams:
+            _streams[device.index] = [torch.cuda.Stream(device=device) for _ in range(2)]
+        return _streams[device.index]
+
+
+def _align(nbytes):
+    return (nbytes + 255) & ~255
+
+
+def _result_views(torch, buf, fofs, nv, nf, kind):
+    """(vertices, faces) of one request: views of its output allocation,
+    the faces from byte ``fofs`` on."""
+    dt, osz = (torch.float32, 4) if kind == MSH_FLOAT32 else (torch.float64, 8)
+    V = buf[:nv * 3 * osz].view(dt).view(nv, 3)
+    F = buf[fofs:fofs + nf * 24].view(torch.int64).view(nf, 3)
+    return V, F
 
 
 @dataclass(frozen=True)
@@ -110,9 +144,8 @@ class Server:
         self.out_guess = out_guess
         self.L = _capi.lib()
         self.slots = []
-        for _ in range(2):
+        for stream in _server_streams(torch, self.device):
             ctx = _capi.Context(device)
-            stream = torch.cuda.Stream(device=self.device)
             ctx.check(self.L.msh_set_stream(ctx.handle, ctypes.c_void_p(stream.cuda_stream)))
             self.slots.append((ctx, stream))
         self.slot = 0
@@ -181,7 +214,7 @@ class Server:
 
     def close(self):
         """Drain the queue (a paused server resumes), finish every request
-        and release the server's contexts and streams."""
+        and release the server's contexts."""
         with self.lock:
             self.paused = False
             self.running = False
@@ -194,6 +227,7 @@ class Server:
         for ctx, stream in self.slots:
             stream.synchronize()
             ctx.close()
+        self.slots = []
 
     def __enter__(self):
         return self
@@ -375,13 +409,17 @@ class Server:
                     t1 = time.perf_counter()
                     vp = (ctypes.c_void_p * n)()
                     fp = (ctypes.c_void_p * n)()
+                    # one allocation per request holds its vertices and
+                    # faces; the completion thread makes the two views
                     for v, r in enumerate(batch):
-                        dt = torch.float32 if kinds[v] == MSH_FLOAT32 else torch.float64
-                        V = torch.empty((nv[v], 3), dtype=dt, device=self.device)
-                        Fc = torch.empty((nf[v], 3), dtype=torch.int64, device=self.device)
-                        vp[v] = V.data_ptr() if nv[v] else None
-                        fp[v] = Fc.data_ptr() if nf[v] else None
-                        r.result = (V, Fc)
+                        osz = 4 if kinds[v] == MSH_FLOAT32 else 8
+                        vb = _align(nv[v] * 3 * osz)
+                        buf = torch.empty(max(vb + nf[v] * 24, 1), dtype=torch.uint8,
+                                          device=self.device)
+                        vp[v] = buf.data_ptr() if nv[v] else None
+                        fp[v] = buf.data_ptr() + vb if nf[v] else None
+                        r.extra["out"] = (buf, vb, nv[v], nf[v], kinds[v])
+                    del buf
                     t2 = time.perf_counter()
                     if n == 1:
                         ctx.check(self.L.msh_extract(ctx.handle, ctypes.byref(F[0]),
@@ -399,7 +437,7 @@ class Server:
                 for r in batch:
                     r.future.set_exception(e)
                     r.result = None
-                    for key in ("field", "desc", "view"):
+                    for key in ("field", "desc", "view", "out"):
                         r.extra.pop(key, None)
                 with self.lock:
                     self.in_use -= sum(r.reserved for r in batch)
@@ -433,6 +471,9 @@ class Server:
                     r.extra.pop(key, None)
                 freed += r.reserved
                 r.future.latency = t - r.t_submit
+                out = r.extra.pop("out", None)
+                if out is not None:
+                    r.result = _result_views(self.torch, *out)
                 cb = r.extra.pop("on_done", None)
                 if cb is None:
                     r.future.set_result(r.result)

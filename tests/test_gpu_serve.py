@@ -216,3 +216,22 @@ def test_oversize_reject_and_cancel(pkg):
     same(r0, alone(pkg, *small[0]))
     for i, r in got.items():
         same(r, alone(pkg, *small[i + 1]))
+
+
+def test_two_servers_at_once_and_batch_views(pkg):
+    # two servers open together share the device's two server streams; each
+    # one's results still equal the extraction alone, also a result kept
+    # after the others of its batch were dropped and their memory reused
+    import torch
+    reqs = requests(pkg, 24, seed=31)
+    with pkg.Server(max_batch=8) as a, pkg.Server(max_batch=8) as b:
+        futs = [(a if v % 2 else b).submit(spec, m) for v, (spec, m) in enumerate(reqs)]
+        results = [f.result(timeout=60) for f in futs]
+    keep = results[5]
+    del results[:5]
+    torch.cuda.empty_cache()
+    junk = torch.full((1 << 22,), -1.0, device="cuda")  # reuses freed memory
+    same(keep, alone(pkg, *reqs[5]))
+    for (spec, m), r in zip(reqs[5:], results):
+        same(r, alone(pkg, spec, m))
+    del junk
