@@ -765,6 +765,7 @@ int msh_batch_count(void* ctx, int32_t n, const msh_field* fields, const msh_par
     const void* df = fields[v].data;
     if (fields[v].memory == MSH_HOST && geoms[v].units > 0) df = (char*)c->fstage + hoff[v];
     VolArgs a = make_vol(b.plans[v], geoms[v], df, w);
+    count_tiling(b.plans[v], &a, false);
     a.coords = w.coords + cbase;
     a.ubase = ubase;
     a.sz = nullptr;  // set below, once the sizes array is placed

@@ -284,7 +284,7 @@ __device__ __forceinline__ void dma_to_lds(__amdgpu_buffer_rsrc_t r, uint32_t* l
 }
 
 // lean.hip: the count kernel of every method and the Marching Cubes emit kernel.
-void count4_tiling(const Plan& p, VolArgs* a);
+void count4_tiling(const Plan& p, VolArgs* a, bool split);
 hipError_t launch_count4(const Plan& p, const VolArgs& a, hipStream_t st);
 hipError_t launch_count4_batch(const Plan& p, const VolArgs* vols, const int32_t* cmap,
                                int64_t gbase, int64_t nblocks, uint32_t* ctr, hipStream_t st);

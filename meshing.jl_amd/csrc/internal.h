@@ -86,6 +86,7 @@ struct VolArgs {
   int64_t* faces;
   int64_t ubase;           // first unit of this volume
   int32_t cnbx, cnby;      // count tiles along x and y
+  int32_t czl;             // log2 of the waves that share a count tile's z segment
   int64_t cntiles;         // count tiles (x * y * z segments)
   int64_t cblk0;           // first count block of this volume in a batched launch
   int32_t nti, ntj;        // emit tiles along x and y
@@ -143,7 +144,10 @@ VolArgs make_vol(const Plan& p, const Geom& g, const void* field, const Workspac
 int emit_variant(const Plan& p, VolArgs* a);
 // the family (Plan::family) a variant belongs to
 int variant_family(int variant);
-void count_tiling(const Plan& p, VolArgs* a);
+// count tiles of a volume; `split`: waves may share a tile's z segment when
+// the volume alone is too small to fill the device (not in a batch, whose
+// persistent launch is kept full by the other volumes)
+void count_tiling(const Plan& p, VolArgs* a, bool split);
 
 hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream_t st);
 hipError_t launch_count(const Plan& p, const VolArgs& a, hipStream_t st);

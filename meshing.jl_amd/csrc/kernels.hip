@@ -815,7 +815,7 @@ int variant_family(int variant) {
   return 0;
 }
 
-void count_tiling(const Plan& p, VolArgs* a) { count4_tiling(p, a); }
+void count_tiling(const Plan& p, VolArgs* a, bool split) { count4_tiling(p, a, split); }
 
 VolArgs make_vol(const Plan& p, const Geom& g, const void* field, const Workspace& w) {
   VolArgs a{};
@@ -838,7 +838,7 @@ VolArgs make_vol(const Plan& p, const Geom& g, const void* field, const Workspac
   a.foff = w.foff;
   a.sz = w.totals;
   a.ubase = 0;
-  count_tiling(p, &a);
+  count_tiling(p, &a, true);
   return a;
 }
 

@@ -37,7 +37,11 @@ __device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, 
 // ---------------------------------------------------------------------------
 // Count.  The 4 waves of a block sit side by side along x; wave w covers
 // sample columns [xb, xb+64) (cells xb..xb+62) and TY cell rows from yb, and
-// marches the z segment of its tile.  Per plane a lane loads TY+1 samples
+// marches the z segment of its tile.  On a volume too small to give every
+// SIMD a few such waves, 2 or 4 waves share one x span instead (a.czl), each
+// marching its own run of whole z blocks, and their sums meet in LDS: a wave
+// is latency-bound (one group of planes in flight behind the other), so
+// shorter runs on more waves finish sooner.  Per plane a lane loads TY+1 samples
 // (buffer loads: the address is a per-row offset fixed for the whole run
 // plus a per-plane base in scalar registers), packs their below-iso bits at
 // even positions, and ORs in the x+1 column's bits from the next lane at
@@ -101,10 +105,24 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
   const int64_t rest = t / nby;
   const int bx = (int)(rest % nbx);
   const int s = (int)(rest / nbx);
-  const int xb = (bx * kWaves + wv) * (kWave - 1);
+  const int czl = a.czl;  // uniform across the block
+  const int wx = wv >> czl, wz = wv & ((1 << czl) - 1);
+  const int xb = (bx * (kWaves >> czl) + wx) * (kWave - 1);
   const int yb = by * TY;
-  if (xb >= g.nx - 1) return;
-  const int zs = s * g.zseg, ze = min(zs + g.zseg, g.nz - 1);
+  const bool xin = xb < g.nx - 1;
+  if (czl == 0 && !xin) return;
+  const int zs0 = s * g.zseg;  // the segment's start: mask bits count from it
+  const int zend = min(zs0 + g.zseg, g.nz - 1);
+  const int zsh = zmask_shift(g.zseg);
+  // this wave's cells [zs, ze): whole z blocks of the segment (a wave
+  // beyond the volume's x end takes none, but meets the block's barrier)
+  int zs = zs0, ze = zend;
+  if (czl) {
+    const int nblk = (zend - zs0 + (1 << zsh) - 1) >> zsh;
+    const int b0 = (wz * nblk) >> czl, b1 = ((wz + 1) * nblk) >> czl;
+    zs = min(zend, zs0 + (b0 << zsh));
+    ze = xin ? min(zend, zs0 + (b1 << zsh)) : zs;
+  }
   const bool xvalid = lane < kWave - 1 && xb + lane < g.nx - 1;
   const int xs = min(xb + lane, g.nx - 1);
   const int64_t pstride = (int64_t)g.nx * g.ny;
@@ -136,7 +154,6 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
   // z blocks where some cell of this lane's TY rows has outputs: a block is
   // marked when the rows' running counts grew across it (the packed sum
   // of a block's growth stays far below 2^32, so it cannot wrap to zero)
-  const int zsh = zmask_shift(g.zseg);
   uint64_t zm = 0;
   uint32_t tprev = 0;
 
@@ -176,11 +193,11 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
   // after the group ending before cell k: close the z block when it ends
   // there (blocks are whole groups; the last one may be short)
   auto close_block = [&](int k) {
-    if (((k - zs) & ((1 << zsh) - 1)) == 0 || k >= ze) {
+    if (((k - zs0) & ((1 << zsh) - 1)) == 0 || k >= ze) {
       uint32_t t = 0;
 #pragma unroll
       for (int r = 0; r < TY; ++r) t += acc[r];
-      zm |= (uint64_t)(t != tprev) << ((k - 1 - zs) >> zsh);
+      zm |= (uint64_t)(t != tprev) << ((k - 1 - zs0) >> zsh);
       tprev = t;
     }
   };
@@ -201,6 +218,27 @@ __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
     use_group(k, vb);
     k += ZU;
     close_block(k);
+  }
+  if (czl) {
+    // the waves of one x span add their rows' counts and OR their masks
+    // (disjoint z blocks); the wave with the first run stores them.  The
+    // next tile of a persistent block writes here only after the tile
+    // queue's barrier, which every wave reaches after its reads below.
+    __shared__ uint32_t pacc[kWaves][TY][kWave];
+    __shared__ uint64_t pzm[kWaves][kWave];
+    if (wz) {
+#pragma unroll
+      for (int r = 0; r < TY; ++r) pacc[wv][r][lane] = acc[r];
+      pzm[wv][lane] = zm;
+    }
+    lds_barrier();
+    if (wz) return;
+    for (int q = 1; q < (1 << czl); ++q) {
+#pragma unroll
+      for (int r = 0; r < TY; ++r) acc[r] += pacc[wv + q][r][lane];
+      zm |= pzm[wv + q][lane];
+    }
+    if (!xin) return;
   }
   if (!xvalid) return;
 #pragma unroll
@@ -562,16 +600,29 @@ __global__ __launch_bounds__(kBlock) void k_emit_mc4_b(const VolArgs* __restrict
 template <typename T>
 static constexpr int count_zu() { return sizeof(T) == 4 ? 2 : 4; }
 
-void count4_tiling(const Plan& p, VolArgs* a) {
+// Count waves with work that keep the device busy (four per SIMD).
+constexpr int64_t kCountWaves = 4096;
+
+void count4_tiling(const Plan& p, VolArgs* a, bool split) {
   const int ty = p.T == K_F32 ? count_ty<float>() : count_ty<double>();
   const int64_t cx = a->g.nx - 1, cy = a->g.ny - 1;
+  a->czl = 0;
   if (cx <= 0 || cy <= 0 || a->g.nzseg <= 0) {
     a->cnbx = a->cnby = 0;
     a->cntiles = 0;
     return;
   }
-  a->cnbx = (int)((cx + kWaves * (kWave - 1) - 1) / (kWaves * (kWave - 1)));
   a->cnby = (int)((cy + ty - 1) / ty);
+  // waves per x span: double them while the volume gives fewer than
+  // kCountWaves waves and each still gets whole z blocks of its own
+  const int64_t spans = (cx + kWave - 2) / (kWave - 1);
+  const int64_t work = spans * a->cnby * a->g.nzseg;
+  const int zblocks = (a->g.zseg + (1 << zmask_shift(a->g.zseg)) - 1) >> zmask_shift(a->g.zseg);
+  int czl = 0;
+  while (split && czl < 2 && (work << (czl + 1)) <= kCountWaves && (1 << (czl + 1)) <= zblocks) ++czl;
+  a->czl = czl;
+  const int per = (kWaves >> czl) * (kWave - 1);  // x cells per count tile
+  a->cnbx = (int)((cx + per - 1) / per);
   a->cntiles = (int64_t)a->cnbx * a->cnby * a->g.nzseg;
 }
 

@@ -123,3 +123,33 @@ def test_autotune_batch_keeps_results(pkg, oracle, reset_families):
     for (f, name), (V, F) in zip(cases, res):
         Vo, Fo = oracle.isosurface(f, name, iso=0)
         assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
+
+
+def waves(shape, seed, dtype):
+    # smooth: long runs of cells without a crossing, so whole z blocks of a
+    # row group stay unmarked
+    rng = np.random.default_rng(seed)
+    x, y, z = np.meshgrid(*[np.linspace(-1, 1, n) for n in shape], indexing="ij", sparse=True)
+    k = rng.uniform(1, 4, 3) * np.pi
+    return (np.cos(k[0] * x) * np.sin(k[1] * y) + np.cos(k[1] * y) * np.sin(k[2] * z) +
+            np.cos(k[2] * z) * np.sin(k[0] * x)).astype(dtype)
+
+
+@pytest.mark.parametrize("name,dtype", CASES)
+def test_count_split_over_z(pkg, oracle, name, dtype):
+    # a volume too small to fill the device on its own has 2 or 4 count waves
+    # share each x span, each over its own whole z blocks: 4 waves over 64-
+    # and 39-cell segments (some waves without blocks) in the first three,
+    # 2 waves with a wave beyond the x end in the last tile in the fourth
+    # (26 M samples: Float32 only); alone and (no sharing) in a batch
+    m = method_of(pkg, name)
+    fields = [waves((300, 120, 200), 21, dtype), waves((151, 260, 104), 22, dtype),
+              waves((40, 40, 1500), 23, dtype)]
+    if dtype == np.float32:
+        fields.append(waves((151, 100, 1729), 24, dtype))
+    outs = [pkg.isosurface(dev_field(f), m) for f in fields]
+    res = pkg.isosurface_batch([dev_field(f) for f in fields], m)
+    for f, (V, F), (Vb, Fb) in zip(fields, outs, res):
+        Vo, Fo = oracle.isosurface(f, name, iso=0)
+        assert_same_mesh(V.cpu().numpy(), F.cpu().numpy(), Vo, Fo)
+        assert_same_mesh(Vb.cpu().numpy(), Fb.cpu().numpy(), Vo, Fo)
