@@ -196,7 +196,8 @@ int msh_generate_field(void* ctx, int32_t kind, int32_t dtype, int64_t nx, int64
  * holding the last call's contents: calls that overlap on one GPU (two
  * contexts, or two streams) race on it and must be serialised by the caller.
  * A volume whose workspace does not fit the array (about 28 bytes per z row
- * of cells, so up to about 1.2 M rows, e.g. 1024 x 1024 x any nz) fails
+ * of cells, each row taken whole up to 4096 cells, so up to about 1.2 M
+ * rows, e.g. 1024 x 1024 x nz for nz up to 4097) fails
  * with MSH_ERR_ARGUMENT, as do host fields.  Results equal msh_extract's. */
 #define MSH_NOALLOC_SCRATCH_BYTES (32u << 20)
 int msh_isosurface_noalloc(void* ctx, const msh_field* field, const msh_params* params,

@@ -129,9 +129,9 @@ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // Julia's promotion for one call (capi.cpp); on error returns an msh_status
 // and, when err is not null, a message in *err.
 int make_plan(std::string* err, const msh_field* f, const msh_params* p, Plan* out);
-// Units of a volume: z rows cut into segments (batch: split only to keep
+// Units of a volume: z rows cut into segments (whole: split only to keep
 // units within 4096 cells).
-Geom make_geom(const msh_field* f, bool batch = false);
+Geom make_geom(const msh_field* f, int method, bool whole = false);
 // Workspace of `units` units and `ncoords` coordinates placed at `base`
 // (null: only the size); returns the bytes it takes.
 size_t workspace_layout(int64_t units, int64_t ncoords, void* base, Workspace* w);

@@ -78,6 +78,14 @@ def test_noalloc_headline_plane_count(pkg):
     m = pkg.MarchingCubes(iso=0)
     V, F, sz = noalloc(pkg, fld, m)
     same((V, F), pkg.isosurface(fld, m))
+    # taller, with tetrahedra: the other entry points cut these z rows into
+    # four segments (four times the workspace, past the scratch); this one
+    # keeps them whole, and the meshes agree
+    fld = pkg.generate_field("sphere", (1024, 1024, 300), "float32")
+    mt = pkg.MarchingTetrahedra(iso=0)
+    V, F, sz = noalloc(pkg, fld, mt)
+    same((V, F), pkg.isosurface(fld, mt))
+    del fld
     # and a small one right after it
     fld = pkg.generate_field("sphere", (20, 21, 22), "float32")
     V, F, sz = noalloc(pkg, fld, m)
