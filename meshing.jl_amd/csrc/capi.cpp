@@ -506,6 +506,7 @@ int msh_create(int device, void** out) {
   // once per device, with the device current: the address of its copy of
   // the static scratch array
   if (e == hipSuccess) e = noalloc_setup(device, &c->scratch);
+  if (e == hipSuccess) e = mt_tables_setup(device, c->own);
   if (e != hipSuccess) {
     if (c->upload) (void)hipEventDestroy(c->upload);
     if (c->pinned) (void)hipHostFree(c->pinned);

@@ -13,6 +13,7 @@
 // cell just below the chunk comes from one extra plane of the brick.
 #include <stdlib.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "device.h"
@@ -25,6 +26,61 @@ namespace {
 
 constexpr int kH5TX = 56;  // cells a block emits along x: lanes 4..59
 constexpr int kH5CY = 4;   // cell rows a block emits along y: one per wave
+
+// The tetrahedra emit's lookup tables in the layout its blocks keep in LDS,
+// by natural case.  Built once per device (mt_tables_setup, at context
+// creation) into a file-scope device copy; every block then copies the
+// 15.5 KB image with 16-byte loads instead of gathering and repacking the
+// constant tables itself (the gather took 12 of 162 us of a 256^3 gyroid
+// emit).
+struct MtTables {
+  uint8_t nown[8 * 256];    // vertices created, by class and (natural) case
+  uint8_t nf[256];          // faces, by case
+  uint16_t face[256 * 12];  // edges of each face, 5 bits each
+  uint8_t rank0[256 * 19];  // rank of each edge among a class-0 voxel's vertices
+  uint8_t own0[256 * 8];    // edges of a class-0 voxel's vertices, in order
+  // owner step of a face vertex, by edge and which of the cells at -x,
+  // -y, -z exist (bits 0-2): the edge's index in its owner | the step
+  // (dx, dy, dz) << 5
+  uint8_t own_step[152];
+  uint8_t pad[8];
+};
+static_assert(sizeof(MtTables) % 16 == 0, "copied in 16-byte words");
+struct NoTables {
+  uint8_t pad[16];
+};
+
+static __device__ __align__(16) MtTables g_mt_tables;
+
+__global__ __launch_bounds__(kBlock) void k_mt_tables() {
+  MtTables& t = g_mt_tables;
+  for (int c = threadIdx.x; c < 256; c += kBlock) {
+    const uint32_t m = nat_to_mt((uint32_t)c);
+#pragma unroll
+    for (int cls = 0; cls < 8; ++cls) t.nown[cls * 256 + c] = MT_NOWN[cls * 256 + m];
+    const int nf = MT_NF[m];
+    t.nf[c] = (uint8_t)nf;
+    for (int q = 0; q < 12; ++q) {
+      uint16_t w = 0;
+      if (q < nf)
+        w = (uint16_t)(MT_FACE[m * 36 + 3 * q] | (MT_FACE[m * 36 + 3 * q + 1] << 5) |
+                       (MT_FACE[m * 36 + 3 * q + 2] << 10));
+      t.face[c * 12 + q] = w;
+    }
+    for (int e = 0; e < 19; ++e) t.rank0[c * 19 + e] = MT_RANK[m * 19 + e];
+    for (int q = 0; q < 8; ++q) t.own0[c * 8 + q] = q < 7 ? MT_OWN_EDGES[m * 20 + q] : 0;
+  }
+  for (int q = threadIdx.x; q < 152; q += kBlock) {
+    // the owner is one cell back along each axis on which the edge's end
+    // corner has coordinate 0, when that cell exists (bits of q & 7)
+    const int ed = q >> 3;
+    const int cb = MT_EDGE[ed] >> 4;
+    const uint32_t d = ((mt_dx(cb) == 0 ? 1u : 0u) | (mt_dy(cb) == 0 ? 2u : 0u) |
+                        (mt_dz(cb) == 0 ? 4u : 0u)) & (uint32_t)(q & 7);
+    t.own_step[q] = (uint8_t)(MT_SHIFT[ed * 8 + d] | (d << 5));
+  }
+  if (threadIdx.x < 8) t.pad[threadIdx.x] = 0;
+}
 
 template <int METHOD, typename T, typename M, typename A, typename Out, int ZC, bool VEC, int CXM,
           int NBUF, class Src>
@@ -42,16 +98,8 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
     uint32_t smp[NBUF][NP * SY * RD];
     double cz[NBUF][NP];
     double cx[SX], cy[SY];
-    // tetrahedra only (size 1 for surface nets)
-    uint8_t nown[K ? 8 * 256 : 1];    // vertices created, by class and (natural) case
-    uint8_t nf[K ? 256 : 1];          // faces, by case
-    uint16_t face[K ? 256 * 12 : 1];  // edges of each face, 5 bits each
-    uint8_t rank0[K ? 256 * 19 : 1];  // rank of each edge among a class-0 voxel's vertices
-    uint8_t own0[K ? 256 * 8 : 1];    // edges of a class-0 voxel's vertices, in order
-    // owner step of a face vertex, by edge and which of the cells at -x,
-    // -y, -z exist (bits 0-2): the edge's index in its owner | the step
-    // (dx, dy, dz) << 5
-    uint8_t own_step[K ? 152 : 1];
+    // tetrahedra only
+    alignas(16) std::conditional_t<K != 0, MtTables, NoTables> mt;
     // in-row vertex prefix << 8 | case of each cell of the chunk (one LDS
     // word per owner look-up)
     uint32_t cpc[NS][K ? ZC : 1][64];
@@ -70,31 +118,9 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
   const int lane = lane_id(), wv = wave_id();
   // lookup tables: the same for every tile
   if constexpr (METHOD == MT) {
-    for (int c = threadIdx.x; c < 256; c += kBlock) {
-      const uint32_t m = nat_to_mt((uint32_t)c);
-#pragma unroll
-      for (int cls = 0; cls < 8; ++cls) sm.nown[cls * 256 + c] = MT_NOWN[cls * 256 + m];
-      const int nf = MT_NF[m];
-      sm.nf[c] = (uint8_t)nf;
-      for (int q = 0; q < 12; ++q) {
-        uint16_t w = 0;
-        if (q < nf)
-          w = (uint16_t)(MT_FACE[m * 36 + 3 * q] | (MT_FACE[m * 36 + 3 * q + 1] << 5) |
-                         (MT_FACE[m * 36 + 3 * q + 2] << 10));
-        sm.face[c * 12 + q] = w;
-      }
-      for (int e = 0; e < 19; ++e) sm.rank0[c * 19 + e] = MT_RANK[m * 19 + e];
-      for (int q = 0; q < 8; ++q) sm.own0[c * 8 + q] = q < 7 ? MT_OWN_EDGES[m * 20 + q] : 0;
-    }
-    for (int q = threadIdx.x; q < 152; q += kBlock) {
-      // the owner is one cell back along each axis on which the edge's end
-      // corner has coordinate 0, when that cell exists (bits of q & 7)
-      const int ed = q >> 3;
-      const int cb = MT_EDGE[ed] >> 4;
-      const uint32_t d = ((mt_dx(cb) == 0 ? 1u : 0u) | (mt_dy(cb) == 0 ? 2u : 0u) |
-                          (mt_dz(cb) == 0 ? 4u : 0u)) & (uint32_t)(q & 7);
-      sm.own_step[q] = (uint8_t)(MT_SHIFT[ed * 8 + d] | (d << 5));
-    }
+    const uint4* __restrict__ src = (const uint4*)&g_mt_tables;
+    uint4* dst = (uint4*)&sm.mt;
+    for (int q = threadIdx.x; q < (int)(sizeof(MtTables) / 16); q += kBlock) dst[q] = src[q];
   }
   if (threadIdx.x < (METHOD == MT ? 19 : 12)) {
     // edge e from corner ca to cb (tetrahedron edges in MT corner order, or
@@ -275,8 +301,8 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
         cs[z] = (exists && z < zn) ? (nib[z + 1] | (nib[z + 2] << 4)) : 0u;
         const uint32_t cls = (uint32_t)((gi == 0) | ((gj == 0) << 1) | ((gk == 0) << 2));
         if constexpr (METHOD == MT) {
-          nvz[z] = sm.nown[cls * 256 + cs[z]];
-          nfz[z] = emits ? (uint32_t)sm.nf[cs[z]] : 0u;
+          nvz[z] = sm.mt.nown[cls * 256 + cs[z]];
+          nfz[z] = emits ? (uint32_t)sm.mt.nf[cs[z]] : 0u;
           sm.cpc[sl][z][lane] = (runv << 8) | cs[z];
         } else {
           const uint32_t csm = nat_to_mc(cs[z]), c0 = csm & 1;
@@ -310,7 +336,7 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
           uint32_t nv;
           if constexpr (METHOD == MT) {
             const uint32_t cls = (uint32_t)((gi == 0) | ((y0 == 0) << 1) | ((zc + z == 0) << 2));
-            nv = sm.nown[cls * 256 + c];
+            nv = sm.mt.nown[cls * 256 + c];
             sm.cpc[0][z][lane] = (hrun << 8) | c;
           } else {
             nv = (c != 0 && c != 255) ? 1u : 0u;
@@ -347,7 +373,7 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
             double px, py, pz;
             if constexpr (METHOD == MT) {
               // class-0 voxels (all but the first layers) read the LDS copy
-              uint32_t ed = sm.own0[ecs * 8 + (q & 7)];
+              uint32_t ed = sm.mt.own0[ecs * 8 + (q & 7)];
               if (cls != 0) ed = MT_OWN_EDGES[(cls * 256 + nat_to_mt(ecs)) * 20 + q];
               const uint32_t ew = sm.edge[ed];
               const T sv0 = *(const T*)(cell + (ew & 0x3FF)), tv = *(const T*)(cell + ((ew >> 10) & 0x3FF));
@@ -413,14 +439,14 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
             const int ci = x0 + lc, cj = gj, ck = zc + c;
             int64_t id[3];
             if constexpr (METHOD == MT) {
-              const uint32_t fw = sm.face[ecs * 12 + q];
+              const uint32_t fw = sm.mt.face[ecs * 12 + q];
               const uint32_t nb = (uint32_t)((ci > 0) | ((cj > 0) << 1) | ((ck > 0) << 2));
   #pragma unroll
               for (int m = 0; m < 3; ++m) {
                 const uint32_t ed = (fw >> (5 * m)) & 31;
                 // the owner (src/marching_tetrahedra.jl:67-84: the first voxel
                 // to request the edge) and the edge's index there
-                const uint32_t st = sm.own_step[ed * 8 + nb];
+                const uint32_t st = sm.mt.own_step[ed * 8 + nb];
                 const uint32_t e2 = st & 31;
                 const int dx = (st >> 5) & 1, dy = (st >> 6) & 1, dz = (st >> 7) & 1;
                 const int l2 = lc - dx, w2 = sl - dy, z2 = c - dz;
@@ -434,9 +460,9 @@ __device__ __forceinline__ void emit_h5_run(Src src) {
                   off = sm.rb[w2][l2] + (int64_t)(w >> 8);
                 } else {
                   oc = sm.pc[w2][l2];
-                  off = sm.rb[w2][l2] - (int64_t)sm.nown[cls2 * 256 + oc];
+                  off = sm.rb[w2][l2] - (int64_t)sm.mt.nown[cls2 * 256 + oc];
                 }
-                uint32_t rk = sm.rank0[oc * 19 + e2];
+                uint32_t rk = sm.mt.rank0[oc * 19 + e2];
                 if (cls2 != 0) rk = MT_RANK[(cls2 * 256 + nat_to_mt(oc)) * 19 + e2];
                 id[m] = off + (int64_t)rk + 1;
               }
@@ -600,6 +626,21 @@ __global__ __launch_bounds__(kBlock) void k_emit_h5_b(const VolArgs* __restrict_
 
 // ---------------------------------------------------------------------------
 // Host side.
+
+hipError_t mt_tables_setup(int device, hipStream_t st) {
+  static std::mutex mu;
+  static bool done[64];
+  if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done[device]) return hipSuccess;
+  // the caller (msh_create) has made `device` current
+  hipLaunchKernelGGL(k_mt_tables, dim3(1), dim3(kBlock), 0, st);
+  hipError_t e = hipGetLastError();
+  // complete before any context's stream can launch an emit that reads it
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) done[device] = true;
+  return e;
+}
 
 // z chunk: 8 cells for surface nets (16 measured slower on dense fields:
 // more chunks overflow the queue) and 4 for tetrahedra, whose LDS tables
