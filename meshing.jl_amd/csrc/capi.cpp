@@ -610,11 +610,16 @@ int msh_count(void* ctx, const msh_field* f, const msh_params* p, int64_t* nvert
     const VolArgs a = make_vol(pl, g, dfield, w);
     hipError_t e = launch_coords(pl, g, w.coords, c->stream);
     if (e == hipSuccess) e = launch_count(pl, a, c->stream);
-    if (e == hipSuccess) e = launch_scan(g, w, c->stream);
-    if (e == hipSuccess)
+    // the scan writes the sizes into the pinned host words itself; a
+    // negative word afterwards (not written) is read back by a copy
+    c->pinned[1] = c->pinned[3] = -1;
+    if (e == hipSuccess) e = launch_scan(g, w, c->stream, c->pinned);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && (c->pinned[1] < 0 || c->pinned[3] < 0)) {
       e = hipMemcpyAsync(c->pinned, w.totals, 4 * sizeof(int64_t), hipMemcpyDeviceToHost,
                          c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
     if (e != hipSuccess) return hip_fail(c, e, "count/scan");
     nv = c->pinned[1];
     nfc = c->pinned[3];

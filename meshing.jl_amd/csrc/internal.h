@@ -151,7 +151,9 @@ void count_tiling(const Plan& p, VolArgs* a, bool split);
 
 hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream_t st);
 hipError_t launch_count(const Plan& p, const VolArgs& a, hipStream_t st);
-hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st);
+// host: pinned host words that also receive the vertex and face totals
+// (at [1] and [3]), or null
+hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st, int64_t* host = nullptr);
 hipError_t launch_emit(const Plan& p, const VolArgs& a, hipStream_t st);
 
 // Batched launches over n volumes whose VolArgs are in device memory.

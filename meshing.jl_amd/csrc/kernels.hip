@@ -148,7 +148,7 @@ constexpr int kScanTilesBlock = 1024;
 
 __global__ __launch_bounds__(kScanTilesBlock) void k_scan_tiles(
     const uint64_t* __restrict__ tilesum, int64_t ntiles, int64_t* __restrict__ tileoff,
-    int64_t* __restrict__ totals) {
+    int64_t* __restrict__ totals, int64_t* __restrict__ host) {
   __shared__ int64_t s_v[kScanTilesBlock / kWave], s_f[kScanTilesBlock / kWave];
   const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
   const int64_t per = (ntiles + kScanTilesBlock - 1) / kScanTilesBlock;
@@ -182,6 +182,13 @@ __global__ __launch_bounds__(kScanTilesBlock) void k_scan_tiles(
     totals[1] = bv + iv;
     totals[2] = 0;
     totals[3] = bf + jf;
+    if (host) {
+      // the sizes straight into the caller's pinned host words (no copy
+      // after the scan), made visible to the host before the kernel ends
+      host[1] = bv + iv;
+      host[3] = bf + jf;
+      __threadfence_system();
+    }
   }
 }
 
@@ -768,12 +775,12 @@ hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st) {
+hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st, int64_t* host) {
   const int64_t nt = scan_tiles(g.units);
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt, g.units,
                      w.tilesum);
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanTilesBlock), 0, st,
-                     (const uint64_t*)w.tilesum, nt, w.tileoff, w.totals);
+                     (const uint64_t*)w.tilesum, nt, w.tileoff, w.totals, host);
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt, g.units,
                      (const int64_t*)w.tileoff, w.voff, w.foff);
   return hipGetLastError();
