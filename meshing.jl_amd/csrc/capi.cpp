@@ -608,12 +608,13 @@ int msh_count(void* ctx, const msh_field* f, const msh_params* p, int64_t* nvert
       dfield = c->fstage;
     }
     const VolArgs a = make_vol(pl, g, dfield, w);
-    hipError_t e = launch_coords(pl, g, w.coords, c->stream);
-    if (e == hipSuccess) e = launch_count(pl, a, c->stream);
-    // the scan writes the sizes into the pinned host words itself; a
-    // negative word afterwards (not written) is read back by a copy
+    hipError_t e = launch_count(pl, a, c->stream);
+    // the scan also fills the coordinate tables and writes the sizes into
+    // the pinned host words itself; a negative word afterwards (not
+    // written) is read back by a copy
     c->pinned[1] = c->pinned[3] = -1;
-    if (e == hipSuccess) e = launch_scan(g, w, c->stream, c->pinned);
+    const CoordArgs ca = coord_args(pl, g, w.coords);
+    if (e == hipSuccess) e = launch_scan(g, w, c->stream, c->pinned, &ca);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && (c->pinned[1] < 0 || c->pinned[3] < 0)) {
       e = hipMemcpyAsync(c->pinned, w.totals, 4 * sizeof(int64_t), hipMemcpyDeviceToHost,

@@ -151,9 +151,19 @@ void count_tiling(const Plan& p, VolArgs* a, bool split);
 
 hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream_t st);
 hipError_t launch_count(const Plan& p, const VolArgs& a, hipStream_t st);
+// The LinRange coordinates of a volume's three axes, end to end, into out.
+struct CoordArgs {
+  double* out;
+  int nx, ny, nz;
+  double a[3], b[3];
+  int mask;  // bit k: axis k in Float32
+};
+CoordArgs coord_args(const Plan& p, const Geom& g, double* coords);
 // host: pinned host words that also receive the vertex and face totals
-// (at [1] and [3]), or null
-hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st, int64_t* host = nullptr);
+// (at [1] and [3]), or null; coords: coordinate tables the scan's one-block
+// kernel also fills (instead of a launch of their own), or null
+hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st, int64_t* host = nullptr,
+                       const CoordArgs* coords = nullptr);
 hipError_t launch_emit(const Plan& p, const VolArgs& a, hipStream_t st);
 
 // Batched launches over n volumes whose VolArgs are in device memory.

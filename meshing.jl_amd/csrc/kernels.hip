@@ -43,18 +43,20 @@ __device__ __forceinline__ double lerpi(int j, int n, double a, double b, bool f
   return f32 ? (double)(float)v : v;
 }
 
-__global__ void k_coords(double* __restrict__ out, int nx, int ny, int nz, double ax, double bx,
-                         double ay, double by, double az, double bz, int cxmask) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nx + ny + nz) return;
-  const double ca[3] = {ax, ay, az}, cb[3] = {bx, by, bz};
-  int axis = 0, n = nx, j = i;
-  if (i >= nx + ny) {
-    axis = 2; n = nz; j = i - nx - ny;
-  } else if (i >= nx) {
-    axis = 1; n = ny; j = i - nx;
+// coordinate i of the three axes laid end to end (x, then y, then z)
+__device__ __forceinline__ double coord_at(const CoordArgs& c, int i) {
+  int axis = 0, n = c.nx, j = i;
+  if (i >= c.nx + c.ny) {
+    axis = 2; n = c.nz; j = i - c.nx - c.ny;
+  } else if (i >= c.nx) {
+    axis = 1; n = c.ny; j = i - c.nx;
   }
-  out[i] = lerpi(j, n, ca[axis], cb[axis], (cxmask >> axis) & 1);
+  return lerpi(j, n, c.a[axis], c.b[axis], (c.mask >> axis) & 1);
+}
+
+__global__ void k_coords(CoordArgs c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < c.nx + c.ny + c.nz) c.out[i] = coord_at(c, i);
 }
 
 // Batched: one block per volume.
@@ -148,7 +150,11 @@ constexpr int kScanTilesBlock = 1024;
 
 __global__ __launch_bounds__(kScanTilesBlock) void k_scan_tiles(
     const uint64_t* __restrict__ tilesum, int64_t ntiles, int64_t* __restrict__ tileoff,
-    int64_t* __restrict__ totals, int64_t* __restrict__ host) {
+    int64_t* __restrict__ totals, int64_t* __restrict__ host, CoordArgs coords) {
+  // the volume's coordinate tables, when asked for: one launch fewer
+  if (coords.out)
+    for (int i = threadIdx.x; i < coords.nx + coords.ny + coords.nz; i += kScanTilesBlock)
+      coords.out[i] = coord_at(coords, i);
   __shared__ int64_t s_v[kScanTilesBlock / kWave], s_f[kScanTilesBlock / kWave];
   const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
   const int64_t per = (ntiles + kScanTilesBlock - 1) / kScanTilesBlock;
@@ -766,21 +772,35 @@ __global__ void k_generate(T* __restrict__ out, int kind, int64_t nx, int64_t ny
 // ---------------------------------------------------------------------------
 // Host launchers.
 
+CoordArgs coord_args(const Plan& p, const Geom& g, double* coords) {
+  CoordArgs c{};
+  c.out = coords;
+  c.nx = g.nx;
+  c.ny = g.ny;
+  c.nz = g.nz;
+  for (int k = 0; k < 3; ++k) {
+    c.a[k] = p.a[k];
+    c.b[k] = p.b[k];
+  }
+  c.mask = (p.cx[0] == K_F32) | ((p.cx[1] == K_F32) << 1) | ((p.cx[2] == K_F32) << 2);
+  return c;
+}
+
 hipError_t launch_coords(const Plan& p, const Geom& g, double* coords, hipStream_t st) {
   const int n = g.nx + g.ny + g.nz;
   if (n == 0) return hipSuccess;
-  const int mask = (p.cx[0] == K_F32) | ((p.cx[1] == K_F32) << 1) | ((p.cx[2] == K_F32) << 2);
-  hipLaunchKernelGGL(k_coords, dim3((n + 255) / 256), dim3(256), 0, st, coords, g.nx, g.ny,
-                     g.nz, p.a[0], p.b[0], p.a[1], p.b[1], p.a[2], p.b[2], mask);
+  hipLaunchKernelGGL(k_coords, dim3((n + 255) / 256), dim3(256), 0, st, coord_args(p, g, coords));
   return hipGetLastError();
 }
 
-hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st, int64_t* host) {
+hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st, int64_t* host,
+                       const CoordArgs* coords) {
   const int64_t nt = scan_tiles(g.units);
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt, g.units,
                      w.tilesum);
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanTilesBlock), 0, st,
-                     (const uint64_t*)w.tilesum, nt, w.tileoff, w.totals, host);
+                     (const uint64_t*)w.tilesum, nt, w.tileoff, w.totals, host,
+                     coords ? *coords : CoordArgs{});
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt, g.units,
                      (const int64_t*)w.tileoff, w.voff, w.foff);
   return hipGetLastError();
