@@ -68,7 +68,9 @@ typedef struct {
   double first[3], last[3];    /* first(X), last(X), ... (Int endpoints as Float64) */
 } msh_params;
 
-/* Context life cycle.  `device` is a HIP device ordinal. */
+/* Context life cycle.  `device` is a HIP device ordinal.  The first context
+ * on a device also builds that device's lookup-table image (one short kernel
+ * on the new context's stream, waited for before msh_create returns). */
 int msh_create(int device, void** ctx);
 int msh_destroy(void* ctx);
 /* Run subsequent work on the caller's hipStream_t; NULL is HIP's default
