@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -431,6 +432,21 @@ size_t workspace_layout(int64_t units, int64_t ncoords, void* base, Workspace* w
   return need;
 }
 
+hipError_t tables_setup(int device, hipStream_t st) {
+  static std::mutex mu;
+  static bool done[64];
+  if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done[device]) return hipSuccess;
+  // the caller (msh_create) has made `device` current
+  hipError_t e = launch_count_tables(st);
+  if (e == hipSuccess) e = launch_mt_tables(st);
+  // complete before any context's stream can launch a kernel that reads them
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) done[device] = true;
+  return e;
+}
+
 }  // namespace msh
 
 namespace {
@@ -506,7 +522,7 @@ int msh_create(int device, void** out) {
   // once per device, with the device current: the address of its copy of
   // the static scratch array
   if (e == hipSuccess) e = noalloc_setup(device, &c->scratch);
-  if (e == hipSuccess) e = mt_tables_setup(device, c->own);
+  if (e == hipSuccess) e = tables_setup(device, c->own);
   if (e != hipSuccess) {
     if (c->upload) (void)hipEventDestroy(c->upload);
     if (c->pinned) (void)hipHostFree(c->pinned);

@@ -13,7 +13,6 @@
 // cell just below the chunk comes from one extra plane of the brick.
 #include <stdlib.h>
 
-#include <mutex>
 #include <type_traits>
 
 #include "device.h"
@@ -28,8 +27,8 @@ constexpr int kH5TX = 56;  // cells a block emits along x: lanes 4..59
 constexpr int kH5CY = 4;   // cell rows a block emits along y: one per wave
 
 // The tetrahedra emit's lookup tables in the layout its blocks keep in LDS,
-// by natural case.  Built once per device (mt_tables_setup, at context
-// creation) into a file-scope device copy; every block then copies the
+// by natural case.  Built once per device (tables_setup in capi.cpp, at
+// context creation) into a file-scope device copy; every block then copies the
 // 15.5 KB image with 16-byte loads instead of gathering and repacking the
 // constant tables itself (the gather took 12 of 162 us of a 256^3 gyroid
 // emit).
@@ -627,19 +626,9 @@ __global__ __launch_bounds__(kBlock) void k_emit_h5_b(const VolArgs* __restrict_
 // ---------------------------------------------------------------------------
 // Host side.
 
-hipError_t mt_tables_setup(int device, hipStream_t st) {
-  static std::mutex mu;
-  static bool done[64];
-  if (device < 0 || device >= 64) return hipErrorInvalidDevice;
-  std::lock_guard<std::mutex> lock(mu);
-  if (done[device]) return hipSuccess;
-  // the caller (msh_create) has made `device` current
+hipError_t launch_mt_tables(hipStream_t st) {
   hipLaunchKernelGGL(k_mt_tables, dim3(1), dim3(kBlock), 0, st);
-  hipError_t e = hipGetLastError();
-  // complete before any context's stream can launch an emit that reads it
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (e == hipSuccess) done[device] = true;
-  return e;
+  return hipGetLastError();
 }
 
 // z chunk: 8 cells for surface nets (16 measured slower on dense fields:

@@ -211,9 +211,12 @@ hipError_t launch_generate(int kind, int dtype, int64_t nx, int64_t ny, int64_t 
 constexpr size_t kScratchBytes = MSH_NOALLOC_SCRATCH_BYTES;
 // This device's address of the scratch array; call with `device` current.
 hipError_t noalloc_setup(int device, void** ptr);
-// Once per device, with it current: the tetrahedra emit's table image
-// (halo5.hip), built on `st` and complete when this returns.
-hipError_t mt_tables_setup(int device, hipStream_t st);
+// Once per device, with it current: the lookup-table images of the count
+// kernels (lean.hip) and the tetrahedra emit (halo5.hip), built on `st` and
+// complete when this returns.
+hipError_t tables_setup(int device, hipStream_t st);
+hipError_t launch_count_tables(hipStream_t st);
+hipError_t launch_mt_tables(hipStream_t st);
 // clamp = (0, min(nverts, vcap), 0, min(nfaces, fcap)) from the scan totals,
 // and sizes (when not null) = (nverts, nfaces); null totals count as zero.
 hipError_t launch_noalloc_sizes(const int64_t* totals, int64_t vcap, int64_t fcap, int64_t* clamp,

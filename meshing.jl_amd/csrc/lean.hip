@@ -50,19 +50,21 @@ template <typename T, int METHOD, int TY, int ZU>
 __device__ __forceinline__ void count4_body(const VolArgs& a, int64_t t,
                                             const uint32_t* __restrict__ lut);
 
-// The count kernels' tile loop: the per-cell count table goes to LDS once
-// per block, then the tiles of `src` are counted one after another.
-template <typename T, int METHOD, int TY, int ZU, class Src>
-__device__ __forceinline__ void count4_run(Src src) {
-  constexpr int NL = METHOD == MC ? 256 : 2048;
-  __shared__ uint32_t lut[NL];  // vertices | faces << 16, by class << 8 | natural case
-  for (int c = threadIdx.x; c < NL; c += kBlock) {
+// The per-cell count tables of the three methods (vertices | faces << 16,
+// by class << 8 | natural case; Marching Cubes has no classes), built once
+// per device (tables_setup in capi.cpp, at context creation); each count block
+// copies its method's table to LDS.
+static __device__ __align__(16) uint32_t g_count_lut[3][2048];
+
+__global__ __launch_bounds__(kBlock) void k_count_tables() {
+  for (int i = threadIdx.x; i < 3 * 2048; i += kBlock) {
+    const int method = i / 2048, c = i % 2048;
     const uint32_t cs = c & 255, cls = c >> 8;
     uint32_t v;
-    if (METHOD == MC) {
+    if (method == MC) {
       const uint32_t m = nat_to_mc(cs);
       v = (uint32_t)(MC_CASE_VERTS[m] & 0xF) | ((uint32_t)(MC_CASE_TRIS[m] & 0x7) << 16);
-    } else if (METHOD == MT) {
+    } else if (method == MT) {
       // cls: first x layer | first y layer << 1 | first z layer << 2
       const uint32_t m = nat_to_mt(cs);
       v = (uint32_t)MT_NOWN[cls * 256 + m] | ((uint32_t)MT_NF[m] << 16);
@@ -79,7 +81,19 @@ __device__ __forceinline__ void count4_run(Src src) {
       }
       v = nv | (nf << 16);
     }
-    lut[c] = v;
+    g_count_lut[method][c] = v;
+  }
+}
+
+// The count kernels' tile loop: the per-cell count table goes to LDS once
+// per block, then the tiles of `src` are counted one after another.
+template <typename T, int METHOD, int TY, int ZU, class Src>
+__device__ __forceinline__ void count4_run(Src src) {
+  constexpr int NL = METHOD == MC ? 256 : 2048;
+  __shared__ __align__(16) uint32_t lut[NL];  // vertices | faces << 16, by class << 8 | natural case
+  {
+    const uint4* __restrict__ from = (const uint4*)g_count_lut[METHOD];
+    for (int q = threadIdx.x; q < NL / 4; q += kBlock) ((uint4*)lut)[q] = from[q];
   }
   __syncthreads();
   if constexpr (Src::kOne) {
@@ -599,6 +613,11 @@ __global__ __launch_bounds__(kBlock) void k_emit_mc4_b(const VolArgs* __restrict
 // Float64 measured best at 1024^3 (Float32 1.14 vs 1.18 ms with 4)
 template <typename T>
 static constexpr int count_zu() { return sizeof(T) == 4 ? 2 : 4; }
+
+hipError_t launch_count_tables(hipStream_t st) {
+  hipLaunchKernelGGL(k_count_tables, dim3(1), dim3(kBlock), 0, st);
+  return hipGetLastError();
+}
 
 // Count waves with work that keep the device busy (four per SIMD).
 constexpr int64_t kCountWaves = 4096;
