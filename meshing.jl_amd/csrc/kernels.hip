@@ -111,7 +111,8 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
 // count, faces in bits 16-31) over tiles of kScanTile units, in three
 // launches that never wait on one another's blocks: k_scan_reduce sums each
 // tile, k_scan_tiles (one block) turns the tile sums into tile offsets and
-// the totals, k_scan_apply scans each tile from its offset.  A single-pass
+// the totals, k_scan_apply scans each tile from its offset (up to
+// kScanFuseTiles tiles the last two are one launch, k_scan_apply_sums).  A single-pass
 // scan with decoupled look-back took 1.1 ms over the 10.9 M units of a
 // 512-volume batch: every tile's inclusive prefix is a round trip through
 // memory that the next tiles wait on.
@@ -226,6 +227,77 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
     rv += c[q] & 0xFFFF;
     rf += c[q] >> 16;
   }
+}
+
+// Up to kScanFuseTiles tiles, k_scan_tiles and k_scan_apply are one launch:
+// each block adds up the sums of the tiles before it itself (at most a few
+// KB of reads), and the last block also writes the totals.  One launch
+// fewer per count phase; past that size the one-block tile scan is
+// cheaper than every block reading the earlier sums.
+constexpr int64_t kScanFuseTiles = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply_sums(
+    const uint32_t* __restrict__ cnt, int64_t units, const uint64_t* __restrict__ tilesum,
+    int64_t* __restrict__ voff, int64_t* __restrict__ foff, int64_t* __restrict__ totals,
+    int64_t* __restrict__ host, CoordArgs coords) {
+  __shared__ uint64_t s_w[kWaves];
+  __shared__ int64_t s_v[kWaves], s_f[kWaves];
+  uint32_t c[kScanItems];
+  const int wv = wave_id(), lane = lane_id();
+  // this tile's offset: the sums of the tiles before it (int64: they
+  // outgrow 32 bits over many tiles)
+  int64_t pv = 0, pf = 0;
+  for (int64_t k = threadIdx.x; k < (int64_t)blockIdx.x; k += kBlock) {
+    pv += (int64_t)(tilesum[k] & 0xFFFFFFFFu);
+    pf += (int64_t)(tilesum[k] >> 32);
+  }
+  const int64_t iv = (int64_t)wave_incl_scan64((uint64_t)pv);
+  const int64_t jf = (int64_t)wave_incl_scan64((uint64_t)pf);
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  const uint64_t sum = scan_load(cnt, units, base, c);
+  const uint64_t wincl = wave_incl_scan64(sum);
+  if (lane == kWave - 1) {
+    s_w[wv] = wincl;
+    s_v[wv] = iv;
+    s_f[wv] = jf;
+  }
+  __syncthreads();
+  uint64_t before = 0, tile = 0;
+  int64_t tv = 0, tf = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) {
+    before += q < wv ? s_w[q] : 0ull;
+    tile += s_w[q];
+    tv += s_v[q];
+    tf += s_f[q];
+  }
+  const uint64_t texcl = before + wincl - sum;  // within the tile: both halves fit
+  int64_t rv = tv + (int64_t)(texcl & 0xFFFFFFFFu);
+  int64_t rf = tf + (int64_t)(texcl >> 32);
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    if (base + q < units) {
+      voff[base + q] = rv;
+      foff[base + q] = rf;
+    }
+    rv += c[q] & 0xFFFF;
+    rf += c[q] >> 16;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    const int64_t nv = tv + (int64_t)(tile & 0xFFFFFFFFu), nf = tf + (int64_t)(tile >> 32);
+    totals[0] = 0;
+    totals[1] = nv;
+    totals[2] = 0;
+    totals[3] = nf;
+    if (host) {
+      host[1] = nv;
+      host[3] = nf;
+      __threadfence_system();
+    }
+  }
+  if (blockIdx.x == 0 && coords.out)
+    for (int i = threadIdx.x; i < coords.nx + coords.ny + coords.nz; i += kBlock)
+      coords.out[i] = coord_at(coords, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -798,6 +870,12 @@ hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st, int64_
   const int64_t nt = scan_tiles(g.units);
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt, g.units,
                      w.tilesum);
+  if (nt <= kScanFuseTiles) {
+    hipLaunchKernelGGL(k_scan_apply_sums, dim3((unsigned)nt), dim3(kBlock), 0, st, w.cnt,
+                       g.units, (const uint64_t*)w.tilesum, w.voff, w.foff, w.totals, host,
+                       coords ? *coords : CoordArgs{});
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanTilesBlock), 0, st,
                      (const uint64_t*)w.tilesum, nt, w.tileoff, w.totals, host,
                      coords ? *coords : CoordArgs{});
