@@ -160,8 +160,9 @@ struct CoordArgs {
 };
 CoordArgs coord_args(const Plan& p, const Geom& g, double* coords);
 // host: pinned host words that also receive the vertex and face totals
-// (at [1] and [3]), or null; coords: coordinate tables the scan's one-block
-// kernel also fills (instead of a launch of their own), or null
+// (at [1] and [3]), or null; coords: coordinate tables the scan also fills
+// (in its one-block tile scan, or block 0 of the fused apply) instead of a
+// launch of their own, or null
 hipError_t launch_scan(const Geom& g, const Workspace& w, hipStream_t st, int64_t* host = nullptr,
                        const CoordArgs* coords = nullptr);
 hipError_t launch_emit(const Plan& p, const VolArgs& a, hipStream_t st);
